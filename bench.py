@@ -1,0 +1,148 @@
+"""Benchmark: B-scan photons/s of the batched persistent kernel vs the plain one-A-scan-per-
+launch path, on the four-layer slab B-scan (786,432 tets, 128 A-scans x 1e6 photons, 512
+bins to 1.5 mm), with the roofline of the transport kernel and the CPU oracle's rate on the
+host cores beside it.
+
+A step is one full B-scan (all A-scans of the workload) through the batched path; the plain
+path is timed once over the same B-scan after the timed steps. Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "oct-mps_amd"))
+
+ALG_BYTES_PER_STEP = 84  # one tet record per photon-step: 4 planes x 16 B + 4 links x 4 B + region 4 B
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E nominal
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scans", type=int, default=128, help="A-scans in the B-scan (workload: 128)")
+    ap.add_argument("--photons", type=int, default=1_000_000, help="photons per A-scan (workload: 1e6)")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-photons", type=int, default=3000, help="oracle photons per sampled A-scan")
+    ap.add_argument("--cpu-scans", type=int, default=8)
+    ap.add_argument("--autotune", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the photon path shards by "
+              f"A-scan with no cross-GPU traffic and is measured at N=1.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import octmps as O
+    from octmps import configs
+
+    t0 = time.time()
+    mesh, params, scans, seed = configs.slab4(n_scans=a.scans, photons=a.photons)
+    print(f"[bench] mesh {mesh.num_tets} tets built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    ctx = O.Context(0)
+    mid = ctx.upload(mesh)
+    opts = O.BatchOpts()
+    opts.use_graph = 1
+    if a.autotune:
+        o, rate = ctx.autotune(mid, params, scans, seed, calib=1 << 21)
+        opts.block_size, opts.refill_quantum = o.block_size, o.refill_quantum
+        print(f"[bench] autotune: block {o.block_size} quantum {o.refill_quantum} ({rate:.3e} photons/s)",
+              file=sys.stderr, flush=True)
+    # Warmup on a small slice of the same B-scan, then W full steps.
+    small = [O.ascan(s.x, s.y, s.z, s.root_tet, min(s.n_photons, 2000), s.stream_salt) for s in scans]
+    ctx.run_bscan(mid, params, small, seed, opts=opts)
+    for i in range(a.warmup):
+        ctx.run_bscan(mid, params, scans, seed, opts=opts)
+        print(f"[bench] warmup step {i} done", file=sys.stderr, flush=True)
+    total_photons = sum(s.n_photons for s in scans)
+    walls, kms, steps = [], [], []
+    for i in range(a.steps):
+        ts = time.perf_counter()
+        t, st = ctx.run_bscan(mid, params, scans, seed, opts=opts)
+        walls.append(time.perf_counter() - ts)
+        kms.append(st.kernel_ms)
+        steps.append(st.photon_steps)
+        print(f"[bench] step {i}: {walls[-1]:.3f}s kernel {st.kernel_ms:.1f} ms, {st.photon_steps} photon-steps, "
+              f"killed {st.killed}", file=sys.stderr, flush=True)
+    wall = sum(walls)
+    value = total_photons * a.steps / wall
+    kernel_s = sum(kms) * 1e-3
+    steps_per_launch = sum(steps) / len(steps)
+    achieved = steps_per_launch * ALG_BYTES_PER_STEP / (kernel_s / a.steps) / 1e9
+    batched_tally = t
+
+    plain = None
+    if not a.no_plain:
+        ts = time.perf_counter()
+        pk, pst = 0.0, 0
+        for s in scans:
+            tp, stp = ctx.run_ascan(mid, params, s, seed)
+            pk += stp.kernel_ms
+            pst += stp.photon_steps
+        pw = time.perf_counter() - ts
+        plain = {"photons_per_s": total_photons / pw, "wall_s": pw, "kernel_ms": pk, "photon_steps": pst}
+        print(f"[bench] plain B-scan: {pw:.3f}s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as OR
+        K = 4096
+        table = OR.Table(O.safeprimes(3 + 3 * K), K)
+        soa = mesh.soa()
+        threads = min(16, os.cpu_count() or 1)
+        idx = np.linspace(0, len(scans) - 1, a.cpu_scans).round().astype(int)
+        ts = time.perf_counter()
+        n = 0
+        for i in idx:
+            s = scans[i]
+            sub = O.ascan(s.x, s.y, s.z, s.root_tet, a.cpu_photons, s.stream_salt)
+            r = OR.run_ascan(soa, params, sub, seed, table, threads=threads)
+            n += r["photons"]
+        cw = time.perf_counter() - ts
+        cpu = {"value": n / cw, "unit": "photons/s", "cores": threads, "kind": "port",
+               "sample": f"{a.cpu_scans} A-scans x {a.cpu_photons} photons of the same B-scan "
+                         f"(FP64 oracle, {threads} threads, {cw:.1f} s)"}
+        print(f"[bench] cpu oracle: {n} photons in {cw:.1f}s", file=sys.stderr, flush=True)
+
+    ctx.close()
+    line = {
+        "metric": "photons/s per B-scan, batched vs plain kernel; p50/p99 A-scan request latency",
+        "value": value,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"slab4 B-scan: 4x4x2 mm four-layer slab, 64x64x32 cells x 6 tets "
+                               f"({mesh.num_tets} tets), {len(scans)} A-scans x {a.photons} photons at 20 um, "
+                               f"512 bins to 1.5 mm, TypeBias 37",
+                   "photons": total_photons, "path": "batched persistent kernel, hipGraph launch"},
+        "batched": {"photons_per_s": value, "kernel_ms": kms, "photon_steps": steps,
+                    "steps_per_photon": steps_per_launch / total_photons},
+        "plain": plain,
+        "batched_over_plain": (value / plain["photons_per_s"]) if plain else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,147 @@
+// Layouts shared by the HIP transport kernel (transport.hip) and the host runtime.
+//
+// Mesh in HBM (structure of arrays, one record per tet, global tet ids across all resident
+// meshes so one launch can walk several meshes):
+//   planes[t*4 + j] : float4 (nx, ny, nz, d), face j pre-signed so n.x + d >= 0 inside
+//                     (64 B per tet, 64-B aligned)
+//   topo[t]         : links[4] neighbour tet or -1, nbr_regions (4 x u8, 0xFF at the mesh
+//                     boundary), region, 2 pad words (32 B per tet)
+// Reference layout being replaced: TetrahedronStructGPU + TriangleFaces (src/octmps.h:52-57,
+// 193-196), read as 4 strided face gathers per step (src/octmps_kernel.cuh:517-545).
+#pragma once
+
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace octmps {
+
+constexpr int kMaxRegions = 100;
+
+struct TopoRec {
+    int32_t link[4];
+    uint32_t nbr_regions;
+    int32_t region;
+    int32_t pad[2];
+};
+static_assert(sizeof(TopoRec) == 32, "TopoRec must be 32 bytes");
+
+// Per-region constants of one parameter set (RegionStructGPU, src/octmps_kernel.h:132-140).
+struct DevRegion {
+    float n;         // refractive index
+    float muas;      // mua + mus (0 for glass)
+    float rmuas;     // 1/(mua+mus) (FLT_MAX for glass, like DBL_MAX in src/octmps_io.cpp:625)
+    float mua_muas;  // mua/(mua+mus)
+    float g;         // anisotropy
+    float hg_a;      // 1 - g^2
+    float hg_b;      // 1/(2g) (0 when g == 0)
+    float lr8;       // (1-g^2) * (sqrt(1+a^2)-1+a) / (2a(1-a) sqrt(1+a^2))
+};
+static_assert(sizeof(DevRegion) == 32, "DevRegion must be 32 bytes");
+
+// Per-parameter-set constants (SimParamGPU, src/octmps_kernel.h:112-130).
+struct DevSet {
+    float n_ambient;
+    float bias_a;        // BackwardBiasCoefficient
+    float td_min, td_max;
+    float p_add;         // ProbabilityAdditionalBias
+    float cos_angle;     // cos(MaxCollectingAngleDeg * pi / 180)
+    float radius2;       // MaxCollectingRadius^2
+    float next_step_num; // -log(rndStepSizeInTissue) (inf for 0)
+    float shift;         // OpticalDepthShift
+    float bin_w;         // CoherenceLengthSource / 6, rounded down
+    float half_lc;       // CoherenceLengthSource / 2
+    int32_t num_bins;
+    int32_t type_bias;
+    float inv_1ma;       // 1/(1-a)
+    float inv_sq;        // 1/sqrt(1+a^2)
+    float inv_2a;        // 1/(2a)
+    float opa2;          // 1 + a^2
+    float pdfb_k;        // sqrt(1+a^2) a (1-a) / (sqrt(1+a^2)-1+a)
+    int32_t region_base; // index of region 0 of this set in the region table
+    int32_t pad;
+};
+static_assert(sizeof(DevSet) == 80, "DevSet must be 80 bytes");
+
+// One A-scan in flight.
+struct DevScan {
+    float px, py, pz;
+    float w0;            // 1 - Rspecular of the root region
+    int32_t root_tet;    // global tet id
+    int32_t root_region;
+    int32_t set;
+    int32_t pad;
+    uint64_t photon_offset;
+    uint64_t salt;
+    uint64_t seed;
+    uint64_t tally_off;  // word offset of this scan's tally in the tally pool
+};
+static_assert(sizeof(DevScan) == 64, "DevScan must be 64 bytes");
+
+// Live photon of one lane saved between slices (the reference's SaveThreadState,
+// src/octmps_kernel.cuh:1154-1224, as one record per lane).
+struct SavedLane {
+    float x, y, z, ux, uy, uz, w, tau, op, maxz, lr, lr_fst, loc_fst;
+    int32_t tet, region;
+    uint32_t flags, steps;
+    uint32_t aG, aR, aS;
+    uint64_t sG, sR, sS;
+    float cx, cy, cz, cux, cuy, cuz, cw, cop, cmaxz;
+    int32_t ctet, cregion;
+    uint32_t slot;
+    uint64_t id;
+    uint32_t ndet;
+    float csum;
+};
+
+struct PhotonLogRec {
+    uint32_t steps;
+    uint32_t detections;
+    float contrib;
+    uint32_t killed;
+};
+
+// Control words.
+enum { CTRL_NEXT_ID = 0, CTRL_STEPS = 1, CTRL_KILLED = 2, CTRL_DONE = 3, CTRL_WORDS = 8 };
+
+struct KArgs {
+    const float4 *planes;
+    const TopoRec *topo;
+    const DevRegion *regions;
+    const DevSet *sets;
+    const DevScan *slots;
+    const uint64_t *seg_start;  // [n_seg + 1] first global photon id of each segment
+    const uint32_t *seg_slot;   // [n_seg] scan slot of each segment
+    const uint64_t *seg_poff;   // [n_seg] photon index (within the scan) at the segment start
+    const uint32_t *mult;       // MWC multiplier table: 3 seeding + 3 per triplet
+    uint64_t *tallies;
+    uint64_t *ctrl;
+    uint64_t *slot_done;        // photons completed per slot
+    SavedLane *save;            // lane states between slices (slice mode)
+    PhotonLogRec *plog;         // optional per-photon summaries, indexed by global id
+    uint64_t id_end;            // ids [ctrl[NEXT], id_end) may be launched
+    uint64_t iter_budget;       // slice mode: loop iterations per wave before saving
+    uint64_t max_steps;         // per-photon step guard
+    int32_t n_seg;
+    uint32_t n_triplets;
+    uint32_t refill_q;
+    uint32_t slice_mode;        // 1: save live photons and exit when ids run out
+    uint32_t restore;           // 1: start from the saved lane states
+    uint32_t exit_on_drain;     // slice mode: leave as soon as the ids run out
+};
+
+// Launchers implemented in transport.hip.
+hipError_t launch_transport(const KArgs &a, int grid, int block, bool multi, bool log,
+                            hipStream_t stream);
+hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t stream);
+hipError_t launch_clear_saved(SavedLane *p, uint64_t n, hipStream_t stream);
+int transport_blocks_per_cu(int block, bool multi, bool log);
+hipError_t launch_probe_rng(const uint32_t *mult, uint32_t n_triplets, uint64_t seed, uint64_t salt,
+                            uint64_t photon0, int n_photons, int n_draws, float *out,
+                            hipStream_t stream);
+hipError_t launch_probe_fresnel(int n, const float *ni, const float *nt, const float *ca1, float *r,
+                                hipStream_t stream);
+hipError_t launch_probe_sampler(int kind, float param, int n, const float *u, float *out,
+                                hipStream_t stream);
+
+}  // namespace octmps

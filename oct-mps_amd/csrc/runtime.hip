@@ -1,0 +1,558 @@
+// Device runtime: context, resident mesh pool, plain and batched runs, hipGraph capture,
+// autotuning, device unit probes.
+//
+// Replaces RunGPUi (src/main.cu:43-660): instead of allocating and uploading the mesh, RNG
+// state and ~40 state arrays for every A-scan and relaunching a fixed-step kernel until a
+// host-read counter reaches zero, a context keeps meshes resident in HBM and runs one
+// persistent transport launch per A-scan (plain) or per B-scan (batched).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device_types.hpp"
+#include "host_common.hpp"
+#include "runtime_internal.hpp"
+
+namespace octmps {
+
+octmps_status hip_fail(hipError_t e, const char *what) {
+    return fail(OCTMPS_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+hipError_t DevBuf::ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) {
+        hipError_t e = hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (e != hipSuccess) return e;
+    }
+    size_t want = std::max<size_t>(n, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Region and set constants of one parameter set (InitDeviceConstantMemory,
+// src/octmps_kernel.cuh:177-220; glass rule of src/octmps_io.cpp:624-626).
+octmps_status make_set(const octmps_params *p, int max_region, int region_base, DevSet &S,
+                       DevRegion *R) {
+    if (p->n_regions < 2 || p->n_regions > (uint32_t)kMaxRegions)
+        return fail(OCTMPS_E_RANGE, "n_regions must be in 2..100 (MAX_REGIONS, src/octmps_kernel.h:110)");
+    if (max_region >= (int)p->n_regions)
+        return fail(OCTMPS_E_RANGE, "mesh uses a region the parameters do not define");
+    if (p->num_depth_bins <= 0 || p->num_depth_bins > (1 << 20))
+        return fail(OCTMPS_E_RANGE, "num_depth_bins out of range");
+    if (!(p->coherence_length_source > 0)) return fail(OCTMPS_E_RANGE, "coherence length must be > 0");
+    const double a = p->backward_bias_coefficient;
+    const double sq = std::sqrt(1.0 + a * a);
+    std::memset(&S, 0, sizeof S);
+    S.n_ambient = (float)p->regions[0].n;
+    S.bias_a = (float)a;
+    S.td_min = (float)p->target_depth_min;
+    S.td_max = (float)p->target_depth_max;
+    S.p_add = (float)p->probability_additional_bias;
+    S.cos_angle = (float)std::cos(p->max_collecting_angle_deg * 3.14159265359 / 180.0);
+    S.radius2 = (float)(p->max_collecting_radius * p->max_collecting_radius);
+    S.next_step_num = (float)(-std::log(p->rnd_step_size_in_tissue));
+    S.shift = (float)p->optical_depth_shift;
+    {
+        // __ddiv_rd(Lc, 6): the round-to-nearest quotient stepped down when too large.
+        double lc = p->coherence_length_source, q = lc / OCTMPS_NUM_SUBSTEPS_RESOLUTION;
+        if (std::fma(-q, (double)OCTMPS_NUM_SUBSTEPS_RESOLUTION, lc) < 0) q = std::nextafter(q, -INFINITY);
+        S.bin_w = (float)q;
+    }
+    S.half_lc = (float)(p->coherence_length_source / 2.0);
+    S.num_bins = p->num_depth_bins;
+    S.type_bias = p->type_bias;
+    S.inv_1ma = (a != 1.0) ? (float)(1.0 / (1.0 - a)) : 0.0f;
+    S.inv_sq = (float)(1.0 / sq);
+    S.inv_2a = (a != 0.0) ? (float)(1.0 / (2.0 * a)) : 0.0f;
+    S.opa2 = (float)(1.0 + a * a);
+    S.pdfb_k = (float)(sq * a * (1.0 - a) / (sq - 1.0 + a));
+    S.region_base = region_base;
+    const double c8 = (a != 0.0 && a != 1.0) ? (sq - 1.0 + a) / (2.0 * a * (1.0 - a) * sq) : 0.0;
+    for (int r = 0; r < kMaxRegions; ++r) {
+        DevRegion &d = R[r];
+        std::memset(&d, 0, sizeof d);
+        if (r == 0) {
+            d.n = (float)p->regions[0].n;
+            continue;
+        }
+        if (r >= (int)p->n_regions) continue;
+        const octmps_region &g = p->regions[r];
+        if (g.mus == 0.0) {  // glass: mutr = DBL_MAX
+            d.muas = 0.0f;
+            d.rmuas = FLT_MAX;
+            d.mua_muas = 0.0f;
+            if (g.mua != 0.0) return fail(OCTMPS_E_RANGE, "glass region (mus = 0) must have mua = 0");
+        } else {
+            double mutr = 1.0 / (g.mua + g.mus);
+            d.muas = (float)(1.0 / mutr);
+            d.rmuas = (float)mutr;
+            d.mua_muas = (float)(g.mua * mutr);
+        }
+        d.n = (float)g.n;
+        d.g = (float)g.g;
+        d.hg_a = (float)(1.0 - g.g * g.g);
+        d.hg_b = (g.g != 0.0) ? (float)(1.0 / (2.0 * g.g)) : 0.0f;
+        d.lr8 = (float)((1.0 - g.g * g.g) * c8);
+    }
+    return OCTMPS_OK;
+}
+
+octmps_status make_scan(octmps_ctx *ctx, const MeshEntry &me, const octmps_params *p,
+                        const octmps_ascan &s, uint64_t seed, int set, uint64_t tally_off,
+                        DevScan &d) {
+    if (s.root_tet < 0 || s.root_tet >= me.n_tets) return fail(OCTMPS_E_PROBE, "root tet out of range");
+    const StreamTable *t = ctx->table;
+    if (!seed_valid(seed, t->mult[0]))
+        return fail(OCTMPS_E_SEED, "seed is not a valid MWC seed (src/octmps_kernel.cuh:86-92)");
+    int rr = me.region[s.root_tet];
+    std::memset(&d, 0, sizeof d);
+    d.px = (float)s.x;
+    d.py = (float)s.y;
+    d.pz = (float)s.z;
+    d.w0 = (float)(1.0 - octmps_rspecular(p->regions[0].n, p->regions[rr].n));
+    d.root_tet = (int32_t)(me.offset + s.root_tet);
+    d.root_region = rr;
+    d.set = set;
+    d.photon_offset = s.photon_offset;
+    d.salt = s.stream_salt;
+    d.seed = seed;
+    d.tally_off = tally_off;
+    return OCTMPS_OK;
+}
+
+int default_block(octmps_ctx *ctx) { return ctx->tuned_block > 0 ? ctx->tuned_block : 256; }
+int default_quantum(octmps_ctx *ctx) { return ctx->tuned_q > 0 ? ctx->tuned_q : 8; }
+
+void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid) {
+    int occ = transport_blocks_per_cu(block, multi, false);
+    if (occ <= 0) occ = 1;
+    int b = blocks_per_cu > 0 ? std::min(blocks_per_cu, occ) : occ;
+    grid = std::max(1, b * ctx->n_cu);
+}
+
+// Runs a list of A-scans to completion in one persistent launch (drain mode).
+octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p, int32_t n,
+                        const octmps_ascan *scans, uint64_t seed, const octmps_batch_opts *opts,
+                        uint64_t *tallies_out, octmps_run_stats *stats, PhotonLogRec *plog_out,
+                        uint64_t plog_n) {
+    double t0 = now_ms();
+    if (!ctx || !p || (n > 0 && !scans) || !tallies_out) return fail(OCTMPS_E_INVALID, "null argument");
+    if (n <= 0) return fail(OCTMPS_E_INVALID, "no A-scans");
+    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    std::lock_guard<std::mutex> pk(ctx->pool_mu);
+    if (mesh_id < 0 || mesh_id >= (int)ctx->meshes.size() || !ctx->meshes[mesh_id].live)
+        return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
+    hipSetDevice(ctx->device);
+    const MeshEntry &me = ctx->meshes[mesh_id];
+    DevSet S;
+    std::vector<DevRegion> R(kMaxRegions);
+    octmps_status st = make_set(p, me.max_region, 0, S, R.data());
+    if (st != OCTMPS_OK) return st;
+    const int32_t B = p->num_depth_bins;
+    const uint64_t words = tally_words(B);
+    std::vector<DevScan> slots(n);
+    std::vector<uint64_t> seg_start, seg_poff;
+    std::vector<uint32_t> seg_slot;
+    uint64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        st = make_scan(ctx, me, p, scans[i], seed, 0, (uint64_t)i * words, slots[i]);
+        if (st != OCTMPS_OK) return st;
+        if (scans[i].n_photons == 0) continue;
+        seg_start.push_back(total);
+        seg_slot.push_back((uint32_t)i);
+        seg_poff.push_back(0);
+        total += scans[i].n_photons;
+    }
+    seg_start.push_back(total);
+    if (seg_slot.empty()) {  // nothing to run: zero tallies
+        std::memset(tallies_out, 0, (size_t)n * words * 8);
+        if (stats) std::memset(stats, 0, sizeof *stats);
+        return OCTMPS_OK;
+    }
+    if (plog_out && plog_n < total) return fail(OCTMPS_E_INVALID, "photon log too small");
+    const int block = (opts && opts->block_size > 0) ? opts->block_size : default_block(ctx);
+    if (block % 64 != 0 || block > 256) return fail(OCTMPS_E_INVALID, "block size must be 64, 128, 192 or 256");
+    const int q = (opts && opts->refill_quantum > 0) ? opts->refill_quantum : default_quantum(ctx);
+    int grid = 0;
+    grid_for(ctx, block, opts ? opts->blocks_per_cu : 0, false, grid);
+    hipError_t e;
+    hipStream_t sm = ctx->stream;
+    WorkBuffers &W = ctx->work;
+    if ((e = W.regions.ensure(R.size() * sizeof(DevRegion))) != hipSuccess ||
+        (e = W.sets.ensure(sizeof(DevSet))) != hipSuccess ||
+        (e = W.slots.ensure(slots.size() * sizeof(DevScan))) != hipSuccess ||
+        (e = W.seg_start.ensure(seg_start.size() * 8)) != hipSuccess ||
+        (e = W.seg_slot.ensure(seg_slot.size() * 4)) != hipSuccess ||
+        (e = W.seg_poff.ensure(seg_poff.size() * 8)) != hipSuccess ||
+        (e = W.tallies.ensure((size_t)n * words * 8)) != hipSuccess ||
+        (e = W.ctrl.ensure(CTRL_WORDS * 8)) != hipSuccess ||
+        (e = W.slot_done.ensure((size_t)n * 8)) != hipSuccess)
+        return hip_fail(e, "device allocation");
+    if (plog_out && (e = W.plog.ensure(total * sizeof(PhotonLogRec))) != hipSuccess)
+        return hip_fail(e, "photon log allocation");
+    if ((e = hipMemcpyAsync(W.regions.p, R.data(), R.size() * sizeof(DevRegion), hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(W.sets.p, &S, sizeof S, hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(W.slots.p, slots.data(), slots.size() * sizeof(DevScan), hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(W.seg_start.p, seg_start.data(), seg_start.size() * 8, hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(W.seg_slot.p, seg_slot.data(), seg_slot.size() * 4, hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(W.seg_poff.p, seg_poff.data(), seg_poff.size() * 8, hipMemcpyHostToDevice, sm)))
+        return hip_fail(e, "upload");
+    KArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.planes = ctx->pool_planes;
+    A.topo = ctx->pool_topo;
+    A.regions = (const DevRegion *)W.regions.p;
+    A.sets = (const DevSet *)W.sets.p;
+    A.slots = (const DevScan *)W.slots.p;
+    A.seg_start = (const uint64_t *)W.seg_start.p;
+    A.seg_slot = (const uint32_t *)W.seg_slot.p;
+    A.seg_poff = (const uint64_t *)W.seg_poff.p;
+    A.mult = ctx->d_mult;
+    A.tallies = (uint64_t *)W.tallies.p;
+    A.ctrl = (uint64_t *)W.ctrl.p;
+    A.slot_done = (uint64_t *)W.slot_done.p;
+    A.plog = plog_out ? (PhotonLogRec *)W.plog.p : nullptr;
+    A.id_end = total;
+    A.max_steps = (opts && opts->max_steps_per_photon) ? opts->max_steps_per_photon : (1ull << 22);
+    A.n_seg = (int32_t)seg_slot.size();
+    A.n_triplets = ctx->table->n_triplets;
+    A.refill_q = (uint32_t)q;
+    const bool use_graph = opts && opts->use_graph && !plog_out;
+    // Graph mode records the events around the graph launch (the fill kernels add a few us);
+    // eager mode records them around the transport kernel alone.
+    auto enqueue = [&](hipStream_t s2, bool timed) -> hipError_t {
+        hipError_t e2;
+        if ((e2 = launch_fill_u64(A.tallies, (uint64_t)n * words, 0, s2)) != hipSuccess) return e2;
+        if ((e2 = launch_fill_u64(A.ctrl, CTRL_WORDS, 0, s2)) != hipSuccess) return e2;
+        if ((e2 = launch_fill_u64(A.slot_done, (uint64_t)n, 0, s2)) != hipSuccess) return e2;
+        if (timed && (e2 = hipEventRecord(ctx->ev0, s2)) != hipSuccess) return e2;
+        if ((e2 = launch_transport(A, grid, block, false, plog_out != nullptr, s2)) != hipSuccess) return e2;
+        return timed ? hipEventRecord(ctx->ev1, s2) : hipSuccess;
+    };
+    if (use_graph) {
+        GraphCache &G = ctx->graph;
+        bool hitc = G.exec && G.grid == grid && G.block == block && std::memcmp(&G.args, &A, sizeof A) == 0;
+        if (!hitc) {
+            if (G.exec) {
+                (void)hipGraphExecDestroy(G.exec);
+                G.exec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            if ((e = hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                return hip_fail(e, "graph capture");
+            hipError_t ec = enqueue(sm, false);
+            e = hipStreamEndCapture(sm, &graph);
+            if (ec != hipSuccess) return hip_fail(ec, "graph capture enqueue");
+            if (e != hipSuccess) return hip_fail(e, "graph capture end");
+            e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+            G.grid = grid;
+            G.block = block;
+            G.args = A;
+        }
+        if ((e = hipEventRecord(ctx->ev0, sm)) != hipSuccess || (e = hipGraphLaunch(G.exec, sm)) != hipSuccess ||
+            (e = hipEventRecord(ctx->ev1, sm)) != hipSuccess)
+            return hip_fail(e, "graph launch");
+    } else {
+        if ((e = enqueue(sm, true)) != hipSuccess) return hip_fail(e, "transport launch");
+    }
+    uint64_t ctrl[CTRL_WORDS];
+    if ((e = hipMemcpyAsync(tallies_out, A.tallies, (size_t)n * words * 8, hipMemcpyDeviceToHost, sm)) ||
+        (e = hipMemcpyAsync(ctrl, A.ctrl, sizeof ctrl, hipMemcpyDeviceToHost, sm)))
+        return hip_fail(e, "download");
+    if (plog_out && (e = hipMemcpyAsync(plog_out, A.plog, total * sizeof(PhotonLogRec), hipMemcpyDeviceToHost, sm)))
+        return hip_fail(e, "download log");
+    if ((e = hipStreamSynchronize(sm)) != hipSuccess) return hip_fail(e, "transport kernel");
+    float kms = 0;
+    (void)hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1);
+    if (stats) {
+        stats->photons = total;
+        stats->photon_steps = ctrl[CTRL_STEPS];
+        stats->killed = ctrl[CTRL_KILLED];
+        stats->kernel_ms = kms;
+        stats->wall_ms = now_ms() - t0;
+        stats->launches = 1;
+        stats->block_size = (uint32_t)block;
+        stats->grid_size = (uint32_t)grid;
+        stats->refill_quantum = (uint32_t)q;
+    }
+    return OCTMPS_OK;
+}
+
+}  // namespace octmps
+
+using namespace octmps;
+
+extern "C" octmps_status octmps_ctx_create(int32_t device, octmps_ctx **out) {
+    if (!out) return fail(OCTMPS_E_INVALID, "null argument");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(OCTMPS_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(OCTMPS_E_DEVICE, "device index out of range");
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const StreamTable *t;
+    octmps_status st = stream_table(&t);
+    if (st != OCTMPS_OK) return st;
+    octmps_ctx *c = new (std::nothrow) octmps_ctx();
+    if (!c) return fail(OCTMPS_E_NOMEM, "host allocation failed");
+    c->device = device;
+    c->table = t;
+    hipDeviceProp_t pr;
+    if ((e = hipGetDeviceProperties(&pr, device)) != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipGetDeviceProperties");
+    }
+    c->n_cu = pr.multiProcessorCount;
+    c->hbm_total = pr.totalGlobalMem;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+        (e = hipMalloc(&c->d_mult, t->mult.size() * 4)) != hipSuccess ||
+        (e = hipMemcpy(c->d_mult, t->mult.data(), t->mult.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) {
+        octmps_ctx_destroy(c);
+        return hip_fail(e, "context setup");
+    }
+    *out = c;
+    return OCTMPS_OK;
+}
+
+extern "C" void octmps_ctx_destroy(octmps_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph.exec) hipGraphExecDestroy(c->graph.exec);
+    c->work.release_all();
+    if (c->pool_planes) hipFree(c->pool_planes);
+    if (c->pool_topo) hipFree(c->pool_topo);
+    if (c->d_mult) hipFree(c->d_mult);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh *mh, int32_t *mesh_id) {
+    if (!c || !mh || !mesh_id) return fail(OCTMPS_E_INVALID, "null argument");
+    const HostMesh &m = mh->m;
+    const int64_t T = m.ntets();
+    if (m.max_region >= 255) return fail(OCTMPS_E_RANGE, "regions must be < 255");
+    std::lock_guard<std::mutex> pk(c->pool_mu);
+    hipSetDevice(c->device);
+    if (c->pool_used + T > (int64_t)INT32_MAX) return fail(OCTMPS_E_RANGE, "mesh pool exceeds 2^31 tets");
+    hipError_t e;
+    if (c->pool_used + T > c->pool_cap) {
+        int64_t ncap = std::max<int64_t>(c->pool_used + T, c->pool_cap * 2);
+        float4 *np = nullptr;
+        TopoRec *nt = nullptr;
+        if ((e = hipMalloc(&np, (size_t)ncap * 4 * sizeof(float4))) != hipSuccess)
+            return fail(OCTMPS_E_NOMEM, "mesh pool allocation failed");
+        if ((e = hipMalloc(&nt, (size_t)ncap * sizeof(TopoRec))) != hipSuccess) {
+            hipFree(np);
+            return fail(OCTMPS_E_NOMEM, "mesh pool allocation failed");
+        }
+        if (c->pool_used > 0) {
+            hipMemcpy(np, c->pool_planes, (size_t)c->pool_used * 4 * sizeof(float4), hipMemcpyDeviceToDevice);
+            hipMemcpy(nt, c->pool_topo, (size_t)c->pool_used * sizeof(TopoRec), hipMemcpyDeviceToDevice);
+        }
+        if (c->pool_planes) hipFree(c->pool_planes);
+        if (c->pool_topo) hipFree(c->pool_topo);
+        c->pool_planes = np;
+        c->pool_topo = nt;
+        c->pool_cap = ncap;
+        if (c->graph.exec) {
+            hipGraphExecDestroy(c->graph.exec);
+            c->graph.exec = nullptr;
+        }
+    }
+    const int64_t off = c->pool_used;
+    std::vector<float4> planes((size_t)T * 4);
+    std::vector<TopoRec> topo((size_t)T);
+    for (int64_t t = 0; t < T; ++t) {
+        for (int j = 0; j < 4; ++j) {
+            const double *p = &m.planes[16 * t + 4 * j];
+            planes[4 * t + j] = make_float4((float)p[0], (float)p[1], (float)p[2], (float)p[3]);
+        }
+        TopoRec &r = topo[t];
+        uint32_t nr = 0;
+        for (int j = 0; j < 4; ++j) {
+            int32_t l = m.links[4 * t + j];
+            r.link[j] = l < 0 ? -1 : (int32_t)(l + off);
+            uint32_t reg = l < 0 ? 0xffu : (uint32_t)m.region[l];
+            nr |= reg << (8 * j);
+        }
+        r.nbr_regions = nr;
+        r.region = m.region[t];
+        r.pad[0] = r.pad[1] = 0;
+    }
+    if ((e = hipMemcpy(c->pool_planes + off * 4, planes.data(), planes.size() * sizeof(float4),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->pool_topo + off, topo.data(), topo.size() * sizeof(TopoRec), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+        return hip_fail(e, "mesh upload");
+    MeshEntry me;
+    me.offset = off;
+    me.n_tets = T;
+    me.max_region = m.max_region;
+    me.region = m.region;
+    me.live = true;
+    c->meshes.push_back(me);
+    c->pool_used += T;
+    *mesh_id = (int32_t)c->meshes.size() - 1;
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_ctx_release_mesh(octmps_ctx *c, int32_t id) {
+    if (!c) return fail(OCTMPS_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> pk(c->pool_mu);
+    if (id < 0 || id >= (int)c->meshes.size() || !c->meshes[id].live) return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
+    c->meshes[id].live = false;
+    c->meshes[id].region.clear();
+    c->meshes[id].region.shrink_to_fit();
+    // The pool range is reclaimed when it is the last one (a stack of meshes).
+    while (!c->meshes.empty() && !c->meshes.back().live && c->meshes.back().offset + c->meshes.back().n_tets == c->pool_used) {
+        c->pool_used = c->meshes.back().offset;
+        c->meshes.back().n_tets = 0;
+        break;
+    }
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_ctx_memory(octmps_ctx *c, uint64_t *used, uint64_t *total) {
+    if (!c) return fail(OCTMPS_E_INVALID, "null argument");
+    uint64_t u = (uint64_t)c->pool_cap * (4 * sizeof(float4) + sizeof(TopoRec)) + c->work.bytes();
+    if (used) *used = u;
+    if (total) *total = c->hbm_total;
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_run_ascan(octmps_ctx *c, int32_t mesh_id, const octmps_params *p,
+                                          const octmps_ascan *scan, uint64_t seed, const octmps_batch_opts *opts,
+                                          uint64_t *tally_out, octmps_run_stats *stats) {
+    return run_scans(c, mesh_id, p, 1, scan, seed, opts, tally_out, stats, nullptr, 0);
+}
+
+extern "C" octmps_status octmps_run_bscan(octmps_ctx *c, int32_t mesh_id, const octmps_params *p, int32_t n,
+                                          const octmps_ascan *scans, uint64_t seed, const octmps_batch_opts *opts,
+                                          uint64_t *tallies_out, octmps_run_stats *stats) {
+    return run_scans(c, mesh_id, p, n, scans, seed, opts, tallies_out, stats, nullptr, 0);
+}
+
+// Runs like octmps_run_bscan and also returns per-photon summaries (steps, detections,
+// summed contribution, killed) for photons in launch order; used by parity tests to pair
+// GPU photons with oracle photons.
+extern "C" octmps_status octmps_run_bscan_logged(octmps_ctx *c, int32_t mesh_id, const octmps_params *p, int32_t n,
+                                                 const octmps_ascan *scans, uint64_t seed,
+                                                 const octmps_batch_opts *opts, uint64_t *tallies_out,
+                                                 octmps_run_stats *stats, void *photon_log, uint64_t log_records) {
+    return run_scans(c, mesh_id, p, n, scans, seed, opts, tallies_out, stats, (PhotonLogRec *)photon_log,
+                     log_records);
+}
+
+extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const octmps_params *p, int32_t n,
+                                         const octmps_ascan *scans, uint64_t seed, uint64_t calib,
+                                         octmps_batch_opts *opts, double *rate) {
+    if (!c || !p || !scans || n <= 0 || !opts) return fail(OCTMPS_E_INVALID, "null argument");
+    if (calib == 0) calib = 1u << 20;
+    // Calibration slice: the first calib/n photons of every A-scan.
+    std::vector<octmps_ascan> cs(scans, scans + n);
+    uint64_t per = std::max<uint64_t>(1, calib / (uint64_t)n);
+    for (auto &s : cs) s.n_photons = std::min<uint64_t>(s.n_photons, per);
+    std::vector<uint64_t> t((size_t)n * tally_words(p->num_depth_bins));
+    const int blocks[] = {64, 128, 256};
+    const int quanta[] = {1, 4, 8, 16, 32};
+    double best = -1;
+    octmps_batch_opts win = *opts;
+    for (int b : blocks)
+        for (int q : quanta) {
+            octmps_batch_opts o = *opts;
+            o.block_size = b;
+            o.refill_quantum = q;
+            o.use_graph = 0;
+            octmps_run_stats st;
+            octmps_status s = run_scans(c, mesh_id, p, n, cs.data(), seed, &o, t.data(), &st, nullptr, 0);
+            if (s != OCTMPS_OK) return s;
+            double r = st.kernel_ms > 0 ? st.photons / (st.kernel_ms * 1e-3) : 0;
+            if (r > best) {
+                best = r;
+                win = o;
+            }
+        }
+    opts->block_size = win.block_size;
+    opts->refill_quantum = win.refill_quantum;
+    c->tuned_block = win.block_size;
+    c->tuned_q = win.refill_quantum;
+    if (rate) *rate = best;
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_probe_rng(octmps_ctx *c, uint64_t seed, uint64_t salt, uint64_t photon0,
+                                          int32_t n_photons, int32_t n_draws, float *out) {
+    if (!c || !out || n_photons <= 0 || n_draws <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
+    std::lock_guard<std::mutex> lk(c->run_mu);
+    hipSetDevice(c->device);
+    size_t nb = (size_t)n_photons * 3 * n_draws * sizeof(float);
+    float *d = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&d, nb)) != hipSuccess) return hip_fail(e, "probe allocation");
+    e = launch_probe_rng(c->d_mult, c->table->n_triplets, seed, salt, photon0, n_photons, n_draws, d, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, nb, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_rng");
+}
+
+extern "C" octmps_status octmps_probe_fresnel(octmps_ctx *c, int32_t n, const float *ni, const float *nt,
+                                              const float *ca1, float *r) {
+    if (!c || n <= 0 || !ni || !nt || !ca1 || !r) return fail(OCTMPS_E_INVALID, "bad probe arguments");
+    std::lock_guard<std::mutex> lk(c->run_mu);
+    hipSetDevice(c->device);
+    float *d = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&d, 4 * (size_t)n * sizeof(float))) != hipSuccess) return hip_fail(e, "probe allocation");
+    hipMemcpyAsync(d, ni, n * 4, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(d + n, nt, n * 4, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(d + 2 * (size_t)n, ca1, n * 4, hipMemcpyHostToDevice, c->stream);
+    e = launch_probe_fresnel(n, d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r, d + 3 * (size_t)n, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_fresnel");
+}
+
+extern "C" octmps_status octmps_probe_sampler(octmps_ctx *c, int32_t kind, float param, int32_t n, const float *u,
+                                              float *out) {
+    if (!c || n <= 0 || !u || !out || (kind != 0 && kind != 1)) return fail(OCTMPS_E_INVALID, "bad probe arguments");
+    std::lock_guard<std::mutex> lk(c->run_mu);
+    hipSetDevice(c->device);
+    float *d = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&d, 2 * (size_t)n * sizeof(float))) != hipSuccess) return hip_fail(e, "probe allocation");
+    hipMemcpyAsync(d, u, n * 4, hipMemcpyHostToDevice, c->stream);
+    e = launch_probe_sampler(kind, param, n, d, d + n, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + n, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_sampler");
+}

@@ -1,0 +1,82 @@
+// Internal state of the device runtime (context, buffers, graph cache).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device_types.hpp"
+#include "host_common.hpp"
+
+namespace octmps {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n);
+    void release();
+};
+
+struct WorkBuffers {
+    DevBuf regions, sets, slots, seg_start, seg_slot, seg_poff, tallies, ctrl, slot_done, plog;
+    void release_all() {
+        for (DevBuf *b : {&regions, &sets, &slots, &seg_start, &seg_slot, &seg_poff, &tallies, &ctrl, &slot_done, &plog})
+            b->release();
+    }
+    uint64_t bytes() const {
+        uint64_t s = 0;
+        for (const DevBuf *b : {&regions, &sets, &slots, &seg_start, &seg_slot, &seg_poff, &tallies, &ctrl, &slot_done, &plog})
+            s += b->cap;
+        return s;
+    }
+};
+
+struct MeshEntry {
+    int64_t offset = 0;   // first global tet id in the pool
+    int64_t n_tets = 0;
+    int32_t max_region = 0;
+    bool live = false;
+    std::vector<int32_t> region;  // host copy of tet regions (root region, Rspecular)
+};
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    int grid = 0, block = 0;
+    KArgs args;
+};
+
+octmps_status hip_fail(hipError_t e, const char *what);
+double now_ms();
+octmps_status make_set(const octmps_params *p, int max_region, int region_base, DevSet &S, DevRegion *R);
+
+}  // namespace octmps
+
+struct octmps_ctx {
+    int device = 0;
+    int n_cu = 0;
+    uint64_t hbm_total = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    const octmps::StreamTable *table = nullptr;
+    uint32_t *d_mult = nullptr;
+    // Resident mesh pool (global tet ids).
+    std::mutex pool_mu;
+    float4 *pool_planes = nullptr;
+    octmps::TopoRec *pool_topo = nullptr;
+    int64_t pool_cap = 0, pool_used = 0;
+    std::vector<octmps::MeshEntry> meshes;
+    // Plain/batched runs.
+    std::mutex run_mu;
+    octmps::WorkBuffers work;
+    octmps::GraphCache graph;
+    int tuned_block = 0, tuned_q = 0;
+};
+
+namespace octmps {
+octmps_status make_scan(octmps_ctx *ctx, const MeshEntry &me, const octmps_params *p, const octmps_ascan &s,
+                        uint64_t seed, int set, uint64_t tally_off, DevScan &d);
+void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid);
+}  // namespace octmps

@@ -1,0 +1,544 @@
+// Serving: continuous batching of A-scan / B-scan requests on one GPU.
+//
+// The reference runs one A-scan at a time from a host loop (src/main.cu:948-1051). Here a
+// scheduler thread keeps meshes resident and runs the transport kernel in slices: before
+// each slice it admits queued requests whose tallies fit the HBM budget and free scan
+// slots, and hands out photon ids round-robin across the A-scans of all admitted requests;
+// during the slice, lanes whose photon dies pick up the next id of any request; at the end
+// of a slice live photons are saved per lane and resumed by the next slice
+// (SaveThreadState/RestoreThreadState, src/octmps_kernel.cuh:1154-1327). Every photon's
+// streams depend only on (seed, salt, photon index), so a request's tallies are bitwise the
+// same alone or in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "device_types.hpp"
+#include "host_common.hpp"
+#include "runtime_internal.hpp"
+
+namespace octmps {
+namespace {
+
+constexpr int kMaxSets = 256;
+
+struct ScanState {
+    int slot = -1;
+    uint64_t n = 0;       // photons to run
+    uint64_t claimed = 0; // photons handed out
+    uint64_t limit = 0;   // n, or claimed once cancelled
+};
+
+struct Req {
+    uint64_t id = 0;
+    int index = 0;
+    int32_t mesh_id = 0;
+    octmps_params p;
+    std::vector<octmps_ascan> scans;
+    uint64_t seed = 0;
+    uint64_t *out = nullptr;
+    uint64_t words = 0;
+    int status = OCTMPS_OK;
+    bool done = false, admitted = false, cancelled = false, fail_once = false;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    uint64_t photons = 0;
+    int set = -1;
+    uint64_t tally_base = 0;
+    std::vector<ScanState> ss;
+};
+
+struct Range {
+    uint64_t off, len;
+};
+
+}  // namespace
+}  // namespace octmps
+
+using namespace octmps;
+
+struct octmps_server {
+    octmps_ctx *ctx = nullptr;
+    octmps_serve_opts o;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    bool stop = false;
+    std::deque<Req *> pending;
+    std::vector<Req *> active;
+    std::map<uint64_t, Req *> reqs;
+    uint64_t next_id = 1;
+    int submits = 0;
+    double t0 = 0;
+    // device state (owned by the scheduler thread)
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DevBuf regions, sets, slots, seg_start, seg_slot, seg_poff, tallies, ctrl, slot_done, save;
+    int grid = 0, block = 0, q = 0;
+    uint64_t lanes = 0;
+    int n_slots = 0;
+    std::vector<int> free_slots, free_sets;
+    std::vector<Range> free_tally;
+    uint64_t tally_cap_words = 0;
+    bool have_saved = false;
+    // stats
+    uint64_t st_photons = 0, st_steps = 0, st_slices = 0;
+    double st_kernel_ms = 0;
+    int fatal = OCTMPS_OK;
+};
+
+namespace octmps {
+namespace {
+
+bool alloc_tally(octmps_server *s, uint64_t words, uint64_t &off) {
+    for (size_t i = 0; i < s->free_tally.size(); ++i) {
+        Range &r = s->free_tally[i];
+        if (r.len >= words) {
+            off = r.off;
+            r.off += words;
+            r.len -= words;
+            if (r.len == 0) s->free_tally.erase(s->free_tally.begin() + i);
+            return true;
+        }
+    }
+    return false;
+}
+
+void free_tally(octmps_server *s, uint64_t off, uint64_t words) {
+    s->free_tally.push_back({off, words});
+    std::sort(s->free_tally.begin(), s->free_tally.end(), [](const Range &a, const Range &b) { return a.off < b.off; });
+    std::vector<Range> m;
+    for (const Range &r : s->free_tally) {
+        if (!m.empty() && m.back().off + m.back().len == r.off) m.back().len += r.len;
+        else m.push_back(r);
+    }
+    s->free_tally.swap(m);
+}
+
+void release(octmps_server *s, Req *r) {
+    for (ScanState &x : r->ss)
+        if (x.slot >= 0) s->free_slots.push_back(x.slot);
+    if (r->set >= 0) s->free_sets.push_back(r->set);
+    if (r->admitted) free_tally(s, r->tally_base, r->words * r->scans.size());
+    r->set = -1;
+    r->admitted = false;
+    for (ScanState &x : r->ss) x.slot = -1;
+}
+
+// Uploads a request's parameter set and scan slots and zeroes its tallies and counters.
+octmps_status stage(octmps_server *s, Req *r) {
+    octmps_ctx *ctx = s->ctx;
+    const MeshEntry &me = ctx->meshes[r->mesh_id];
+    DevSet S;
+    std::vector<DevRegion> R(kMaxRegions);
+    octmps_status st = make_set(&r->p, me.max_region, r->set * kMaxRegions, S, R.data());
+    if (st != OCTMPS_OK) return st;
+    hipError_t e;
+    if ((e = hipMemcpyAsync((DevRegion *)s->regions.p + (size_t)r->set * kMaxRegions, R.data(),
+                            R.size() * sizeof(DevRegion), hipMemcpyHostToDevice, s->stream)) ||
+        (e = hipMemcpyAsync((DevSet *)s->sets.p + r->set, &S, sizeof S, hipMemcpyHostToDevice, s->stream)))
+        return hip_fail(e, "serve stage");
+    for (size_t i = 0; i < r->scans.size(); ++i) {
+        DevScan d;
+        st = make_scan(ctx, me, &r->p, r->scans[i], r->seed, r->set, r->tally_base + i * r->words, d);
+        if (st != OCTMPS_OK) return st;
+        int slot = r->ss[i].slot;
+        if ((e = hipMemcpyAsync((DevScan *)s->slots.p + slot, &d, sizeof d, hipMemcpyHostToDevice, s->stream)) ||
+            (e = launch_fill_u64((uint64_t *)s->slot_done.p + slot, 1, 0, s->stream)))
+            return hip_fail(e, "serve stage");
+    }
+    if ((e = launch_fill_u64((uint64_t *)s->tallies.p + r->tally_base, r->words * r->scans.size(), 0, s->stream)))
+        return hip_fail(e, "serve stage");
+    for (ScanState &x : r->ss) {
+        x.claimed = 0;
+        x.limit = x.n;
+    }
+    return OCTMPS_OK;
+}
+
+// Admission under the lock: FIFO while slots, a parameter set and tally space are free.
+void admit(octmps_server *s, std::vector<Req *> &staged) {
+    while (!s->pending.empty()) {
+        Req *r = s->pending.front();
+        if (r->cancelled) {
+            s->pending.pop_front();
+            r->status = OCTMPS_E_CANCELLED;
+            r->done = true;
+            r->t_done = now_ms() - s->t0;
+            s->cv_done.notify_all();
+            continue;
+        }
+        size_t need = r->scans.size();
+        if (s->free_slots.size() < need || s->free_sets.empty()) break;
+        uint64_t off;
+        if (!alloc_tally(s, r->words * need, off)) break;
+        r->tally_base = off;
+        r->admitted = true;
+        r->set = s->free_sets.back();
+        s->free_sets.pop_back();
+        for (size_t i = 0; i < need; ++i) {
+            r->ss[i].slot = s->free_slots.back();
+            s->free_slots.pop_back();
+        }
+        r->t_admit = now_ms() - s->t0;
+        s->pending.pop_front();
+        s->active.push_back(r);
+        staged.push_back(r);
+    }
+}
+
+void finish(octmps_server *s, Req *r, int status) {
+    r->status = status;
+    r->done = true;
+    r->t_done = now_ms() - s->t0;
+    release(s, r);
+    s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+    s->cv_done.notify_all();
+}
+
+void scheduler(octmps_server *s) {
+    octmps_ctx *ctx = s->ctx;
+    hipSetDevice(ctx->device);
+    std::vector<uint64_t> done_h((size_t)s->n_slots);
+    for (;;) {
+        std::vector<Req *> staged;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            while (!s->stop && s->pending.empty() && s->active.empty()) s->cv_work.wait(lk);
+            if (s->stop && s->pending.empty() && s->active.empty()) break;
+            admit(s, staged);
+        }
+        std::lock_guard<std::mutex> pk(ctx->pool_mu);
+        for (Req *r : staged) {
+            octmps_status st = stage(s, r);
+            if (st != OCTMPS_OK) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                finish(s, r, st);
+            }
+        }
+        // Segments: round-robin chunks over A-scans that still have photons to hand out.
+        std::vector<uint64_t> seg_start, seg_poff;
+        std::vector<uint32_t> seg_slot;
+        std::vector<std::pair<Req *, int>> seg_owner;
+        std::vector<std::pair<Req *, int>> open;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (Req *r : s->active)
+                for (size_t i = 0; i < r->ss.size(); ++i)
+                    if (r->ss[i].claimed < r->ss[i].limit) open.push_back({r, (int)i});
+        }
+        uint64_t budget = s->o.slice_photons ? s->o.slice_photons : s->lanes * 4;
+        uint64_t total = 0;
+        if (!open.empty()) {
+            uint64_t chunk = std::max<uint64_t>(256, budget / open.size());
+            std::vector<uint64_t> cur(open.size());
+            for (size_t k = 0; k < open.size(); ++k) cur[k] = open[k].first->ss[open[k].second].claimed;
+            bool any = true;
+            while (any && total < budget) {
+                any = false;
+                for (size_t k = 0; k < open.size() && total < budget; ++k) {
+                    ScanState &x = open[k].first->ss[open[k].second];
+                    if (cur[k] >= x.limit) continue;
+                    uint64_t len = std::min<uint64_t>(chunk, x.limit - cur[k]);
+                    seg_start.push_back(total);
+                    seg_poff.push_back(cur[k]);
+                    seg_slot.push_back((uint32_t)x.slot);
+                    seg_owner.push_back(open[k]);
+                    cur[k] += len;
+                    total += len;
+                    any = true;
+                }
+            }
+        }
+        seg_start.push_back(total);
+        if (seg_slot.empty()) {  // keep the table non-empty for the binary search
+            seg_slot.push_back(0);
+            seg_poff.push_back(0);
+            seg_start.insert(seg_start.begin(), 0);
+        }
+        hipError_t e;
+        bool ok = (e = s->seg_start.ensure(seg_start.size() * 8)) == hipSuccess &&
+                  (e = s->seg_slot.ensure(seg_slot.size() * 4)) == hipSuccess &&
+                  (e = s->seg_poff.ensure(seg_poff.size() * 8)) == hipSuccess &&
+                  (e = hipMemcpyAsync(s->seg_start.p, seg_start.data(), seg_start.size() * 8, hipMemcpyHostToDevice,
+                                      s->stream)) == hipSuccess &&
+                  (e = hipMemcpyAsync(s->seg_slot.p, seg_slot.data(), seg_slot.size() * 4, hipMemcpyHostToDevice,
+                                      s->stream)) == hipSuccess &&
+                  (e = hipMemcpyAsync(s->seg_poff.p, seg_poff.data(), seg_poff.size() * 8, hipMemcpyHostToDevice,
+                                      s->stream)) == hipSuccess &&
+                  (e = launch_fill_u64((uint64_t *)s->ctrl.p, CTRL_WORDS, 0, s->stream)) == hipSuccess;
+        KArgs A;
+        std::memset(&A, 0, sizeof A);
+        A.planes = ctx->pool_planes;
+        A.topo = ctx->pool_topo;
+        A.regions = (const DevRegion *)s->regions.p;
+        A.sets = (const DevSet *)s->sets.p;
+        A.slots = (const DevScan *)s->slots.p;
+        A.seg_start = (const uint64_t *)s->seg_start.p;
+        A.seg_slot = (const uint32_t *)s->seg_slot.p;
+        A.seg_poff = (const uint64_t *)s->seg_poff.p;
+        A.mult = ctx->d_mult;
+        A.tallies = (uint64_t *)s->tallies.p;
+        A.ctrl = (uint64_t *)s->ctrl.p;
+        A.slot_done = (uint64_t *)s->slot_done.p;
+        A.save = (SavedLane *)s->save.p;
+        A.id_end = total;
+        A.iter_budget = 4096;
+        A.max_steps = 1ull << 22;
+        A.n_seg = (int32_t)seg_slot.size();
+        A.n_triplets = ctx->table->n_triplets;
+        A.refill_q = (uint32_t)s->q;
+        A.slice_mode = 1;
+        A.restore = s->have_saved ? 1u : 0u;
+        A.exit_on_drain = total > 0 ? 1u : 0u;
+        uint64_t ctrl[CTRL_WORDS] = {0};
+        if (ok) ok = (e = hipEventRecord(s->e0, s->stream)) == hipSuccess &&
+                     (e = launch_transport(A, s->grid, s->block, true, false, s->stream)) == hipSuccess &&
+                     (e = hipEventRecord(s->e1, s->stream)) == hipSuccess &&
+                     (e = hipMemcpyAsync(ctrl, s->ctrl.p, sizeof ctrl, hipMemcpyDeviceToHost, s->stream)) == hipSuccess &&
+                     (e = hipMemcpyAsync(done_h.data(), s->slot_done.p, done_h.size() * 8, hipMemcpyDeviceToHost,
+                                         s->stream)) == hipSuccess &&
+                     (e = hipStreamSynchronize(s->stream)) == hipSuccess;
+        if (!ok) {
+            hip_fail(e, "serve slice");
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->fatal = OCTMPS_E_DEVICE;
+            while (!s->active.empty()) finish(s, s->active.front(), OCTMPS_E_DEVICE);
+            while (!s->pending.empty()) {
+                Req *r = s->pending.front();
+                s->pending.pop_front();
+                r->status = OCTMPS_E_DEVICE;
+                r->done = true;
+                s->cv_done.notify_all();
+            }
+            break;
+        }
+        s->have_saved = true;
+        float kms = 0;
+        hipEventElapsedTime(&kms, s->e0, s->e1);
+        s->st_kernel_ms += kms;
+        s->st_steps += ctrl[CTRL_STEPS];
+        s->st_slices++;
+        // Ids are claimed in increasing order: every segment below `claimed` was handed out.
+        uint64_t claimed = std::min<uint64_t>(ctrl[CTRL_NEXT_ID], total);
+        for (size_t k = 0; k < seg_owner.size(); ++k) {
+            uint64_t a = seg_start[k], b = seg_start[k + 1];
+            if (claimed <= a) break;
+            uint64_t used = std::min(claimed, b) - a;
+            seg_owner[k].first->ss[seg_owner[k].second].claimed += used;
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        std::vector<Req *> act = s->active;
+        for (Req *r : act) {
+            bool complete = true;
+            uint64_t ph = 0;
+            for (ScanState &x : r->ss) {
+                uint64_t d = done_h[(size_t)x.slot];
+                ph += d;
+                if (!(x.claimed >= x.limit && d >= x.claimed)) complete = false;
+            }
+            if (!complete) continue;
+            if (r->cancelled) {
+                finish(s, r, OCTMPS_E_CANCELLED);
+                continue;
+            }
+            if (r->fail_once) {
+                // Injected device fault on this work item: discard and rerun it from its
+                // deterministic streams in the same slots.
+                r->fail_once = false;
+                stage(s, r);
+                continue;
+            }
+            hipError_t ec = hipMemcpy(r->out, (uint64_t *)s->tallies.p + r->tally_base,
+                                      r->words * r->scans.size() * 8, hipMemcpyDeviceToHost);
+            r->photons = ph;
+            s->st_photons += ph;
+            finish(s, r, ec == hipSuccess ? OCTMPS_OK : OCTMPS_E_DEVICE);
+        }
+    }
+}
+
+}  // namespace
+}  // namespace octmps
+
+extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_opts *opts, octmps_server **out) {
+    if (!ctx || !out) return fail(OCTMPS_E_INVALID, "null argument");
+    octmps_server *s = new (std::nothrow) octmps_server();
+    if (!s) return fail(OCTMPS_E_NOMEM, "host allocation failed");
+    std::memset(&s->o, 0, sizeof s->o);
+    s->o.fail_request_index = -1;
+    if (opts) s->o = *opts;
+    s->ctx = ctx;
+    s->t0 = now_ms();
+    s->n_slots = s->o.max_inflight_scans > 0 ? s->o.max_inflight_scans : 4096;
+    s->block = s->o.block_size > 0 ? s->o.block_size : 256;
+    s->q = s->o.refill_quantum > 0 ? s->o.refill_quantum : 8;
+    if (s->block % 64 != 0 || s->block > 256) {
+        delete s;
+        return fail(OCTMPS_E_INVALID, "block size must be 64, 128, 192 or 256");
+    }
+    hipSetDevice(ctx->device);
+    grid_for(ctx, s->block, 0, true, s->grid);
+    s->lanes = (uint64_t)s->grid * s->block;
+    uint64_t budget = s->o.hbm_budget_bytes ? s->o.hbm_budget_bytes : (1ull << 30);
+    s->tally_cap_words = budget / 8;
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&s->e0)) != hipSuccess || (e = hipEventCreate(&s->e1)) != hipSuccess ||
+        (e = s->regions.ensure((size_t)kMaxSets * kMaxRegions * sizeof(DevRegion))) != hipSuccess ||
+        (e = s->sets.ensure((size_t)kMaxSets * sizeof(DevSet))) != hipSuccess ||
+        (e = s->slots.ensure((size_t)s->n_slots * sizeof(DevScan))) != hipSuccess ||
+        (e = s->slot_done.ensure((size_t)s->n_slots * 8)) != hipSuccess ||
+        (e = s->ctrl.ensure(CTRL_WORDS * 8)) != hipSuccess ||
+        (e = s->tallies.ensure(s->tally_cap_words * 8)) != hipSuccess ||
+        (e = s->save.ensure(s->lanes * sizeof(SavedLane))) != hipSuccess ||
+        (e = launch_clear_saved((SavedLane *)s->save.p, s->lanes, s->stream)) != hipSuccess ||
+        (e = launch_fill_u64((uint64_t *)s->slot_done.p, (uint64_t)s->n_slots, 0, s->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(s->stream)) != hipSuccess) {
+        octmps_serve_close(s);
+        return fail(OCTMPS_E_NOMEM, std::string("server allocation: ") + hipGetErrorString(e));
+    }
+    for (int i = s->n_slots - 1; i >= 0; --i) s->free_slots.push_back(i);
+    for (int i = kMaxSets - 1; i >= 0; --i) s->free_sets.push_back(i);
+    s->free_tally.push_back({0, s->tally_cap_words});
+    s->th = std::thread(scheduler, s);
+    *out = s;
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_serve_submit(octmps_server *s, const octmps_request *rq, uint64_t *req_id) {
+    if (!s || !rq || !rq->params || !rq->scans || rq->n_scans <= 0 || !rq->tallies_out)
+        return fail(OCTMPS_E_INVALID, "bad request");
+    octmps_ctx *ctx = s->ctx;
+    {
+        std::lock_guard<std::mutex> pk(ctx->pool_mu);
+        if (rq->mesh_id < 0 || rq->mesh_id >= (int)ctx->meshes.size() || !ctx->meshes[rq->mesh_id].live)
+            return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
+        DevSet S;
+        std::vector<DevRegion> R(kMaxRegions);
+        octmps_status st = make_set(rq->params, ctx->meshes[rq->mesh_id].max_region, 0, S, R.data());
+        if (st != OCTMPS_OK) return st;
+        for (int i = 0; i < rq->n_scans; ++i)
+            if (rq->scans[i].root_tet < 0 || rq->scans[i].root_tet >= ctx->meshes[rq->mesh_id].n_tets)
+                return fail(OCTMPS_E_PROBE, "root tet out of range");
+        if (!seed_valid(rq->seed, ctx->table->mult[0])) return fail(OCTMPS_E_SEED, "invalid seed");
+    }
+    Req *r = new (std::nothrow) Req();
+    if (!r) return fail(OCTMPS_E_NOMEM, "host allocation failed");
+    r->mesh_id = rq->mesh_id;
+    r->p = *rq->params;
+    r->scans.assign(rq->scans, rq->scans + rq->n_scans);
+    r->seed = rq->seed;
+    r->out = rq->tallies_out;
+    r->words = tally_words(rq->params->num_depth_bins);
+    r->ss.resize((size_t)rq->n_scans);
+    for (int i = 0; i < rq->n_scans; ++i) r->ss[i].n = r->ss[i].limit = rq->scans[i].n_photons;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) {
+        delete r;
+        return fail(OCTMPS_E_BUSY, "server is closing");
+    }
+    if (s->fatal != OCTMPS_OK) {
+        delete r;
+        return fail((octmps_status)s->fatal, "server stopped after a device error");
+    }
+    if ((uint64_t)rq->n_scans > (uint64_t)s->n_slots || r->words * rq->n_scans > s->tally_cap_words) {
+        delete r;
+        return fail(OCTMPS_E_NOMEM, "request exceeds the server's slots or HBM budget");
+    }
+    r->id = s->next_id++;
+    r->index = s->submits++;
+    r->fail_once = (r->index == s->o.fail_request_index);
+    r->t_submit = now_ms() - s->t0;
+    s->reqs[r->id] = r;
+    s->pending.push_back(r);
+    if (req_id) *req_id = r->id;
+    s->cv_work.notify_all();
+    return OCTMPS_OK;
+}
+
+static void fill_info(const Req *r, octmps_request_info *info) {
+    if (!info) return;
+    info->status = r->status;
+    info->done = r->done ? 1 : 0;
+    info->t_submit_ms = r->t_submit;
+    info->t_admit_ms = r->t_admit;
+    info->t_done_ms = r->t_done;
+    info->photons = r->photons;
+}
+
+extern "C" octmps_status octmps_serve_poll(octmps_server *s, uint64_t id, octmps_request_info *info) {
+    if (!s) return fail(OCTMPS_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(OCTMPS_E_NOT_FOUND, "unknown request");
+    fill_info(it->second, info);
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_serve_wait(octmps_server *s, uint64_t id, double timeout_s, octmps_request_info *info) {
+    if (!s) return fail(OCTMPS_E_INVALID, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(OCTMPS_E_NOT_FOUND, "unknown request");
+    Req *r = it->second;
+    auto pred = [&] { return r->done; };
+    if (timeout_s < 0) s->cv_done.wait(lk, pred);
+    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) {
+        fill_info(r, info);
+        return fail(OCTMPS_E_TIMEOUT, "request still running");
+    }
+    fill_info(r, info);
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_serve_cancel(octmps_server *s, uint64_t id) {
+    if (!s) return fail(OCTMPS_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(OCTMPS_E_NOT_FOUND, "unknown request");
+    Req *r = it->second;
+    if (r->done) return OCTMPS_OK;
+    r->cancelled = true;
+    for (ScanState &x : r->ss) x.limit = x.claimed;  // hand out nothing more
+    s->cv_work.notify_all();
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_serve_stats(octmps_server *s, uint64_t *photons, uint64_t *steps, double *kernel_ms,
+                                            uint64_t *slices) {
+    if (!s) return fail(OCTMPS_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (photons) *photons = s->st_photons;
+    if (steps) *steps = s->st_steps;
+    if (kernel_ms) *kernel_ms = s->st_kernel_ms;
+    if (slices) *slices = s->st_slices;
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_serve_close(octmps_server *s) {
+    if (!s) return fail(OCTMPS_E_INVALID, "null argument");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->th.joinable()) s->th.join();
+    hipSetDevice(s->ctx->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (DevBuf *b : {&s->regions, &s->sets, &s->slots, &s->seg_start, &s->seg_slot, &s->seg_poff, &s->tallies,
+                      &s->ctrl, &s->slot_done, &s->save})
+        b->release();
+    if (s->e0) hipEventDestroy(s->e0);
+    if (s->e1) hipEventDestroy(s->e1);
+    if (s->stream) hipStreamDestroy(s->stream);
+    for (auto &kv : s->reqs) delete kv.second;
+    delete s;
+    return OCTMPS_OK;
+}

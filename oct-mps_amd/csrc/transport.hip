@@ -1,0 +1,640 @@
+// Photon transport kernel for gfx950 (CDNA4, wave64).
+//
+// One persistent kernel runs the reference's per-photon loop (OCTMPSKernel,
+// src/octmps_kernel.cuh:1329-1402) for any mix of A-scans: each lane holds one photon in
+// VGPRs; when enough lanes of a wave have lost their photon (refill quantum), the wave claims
+// that many photon ids with one atomic and relaunches those lanes (ballot + mbcnt ranks).
+// Photon ids map to A-scans through a sorted segment table, and every photon derives its
+// three MWC streams from (seed, salt, photon index), so results do not depend on lane,
+// launch or slice. Transport is FP32; tallies are exact Q64.64 integers (two 64-bit words
+// with a carry, like the reference's EMULATED_ATOMIC, src/octmps_kernel.cuh:42-46).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "device_types.hpp"
+#include "stream_init.hpp"
+
+namespace octmps {
+namespace {
+
+// FP32 thresholds. The reference's COSZERO = 1 - 1e-14 (src/octmps.h:29) rounds to 1.0f,
+// which would send exact normal incidence down the 0/0 Fresnel branch, so FP32 uses
+// 1 - 1e-6; a direction is "normal" for the rotation when ux^2 + uy^2 <= 1e-12.
+constexpr float kCosZeroF = 1.0f - 1e-6f;
+constexpr float kCos90F = 1e-6f;        // COSNINETYDEG, src/octmps.h:28
+constexpr float kWeightF = 1e-4f;       // WEIGHT, src/octmps.h:22
+constexpr float kChanceF = 0.1f;        // CHANCE, src/octmps.h:23
+constexpr float kNormalSin2 = 1e-12f;
+
+constexpr uint32_t F_ALIVE = 1u, F_FST = 2u, F_NBS = 4u, F_CNBS = 8u;
+
+// __uint2float_rz(u) / 2^32: keep the top 24 significant bits, then an exact scale.
+__device__ __forceinline__ float unit_rz24(uint32_t u) {
+    int shift = 8 - __clz((int)u);
+    shift = shift < 0 ? 0 : shift;
+    uint32_t t = u & (0xffffffffu << shift);
+    return (float)t * 2.3283064365386963e-10f;
+}
+
+// rand_MWC_co / rand_MWC_oc, src/octmps_kernel.cuh:10-29.
+__device__ __forceinline__ float rand_co(uint64_t &x, uint32_t a) {
+    x = (x & 0xffffffffull) * (uint64_t)a + (x >> 32);
+    return unit_rz24((uint32_t)x);
+}
+__device__ __forceinline__ float rand_oc(uint64_t &x, uint32_t a) { return 1.0f - rand_co(x, a); }
+
+// Fresnel reflectance of FastReflectTransmit, src/octmps_kernel.cuh:608-645.
+__device__ __forceinline__ float fresnel_f(float ni, float nt, float ca1) {
+    float ni_nt = ni / nt;
+    bool normal = ca1 > kCosZeroF;
+    float sa1 = normal ? 0.0f : sqrtf(fmaxf(1.0f - ca1 * ca1, 0.0f));
+    float sa2 = fminf(ni_nt * sa1, 1.0f);
+    float ca2 = sqrtf(fmaxf(1.0f - sa2 * sa2, 0.0f));
+    float ca1ca2 = ca1 * ca2, sa1sa2 = sa1 * sa2;
+    float sa1ca2 = normal ? 1.0f : sa1 * ca2;
+    float ca1sa2 = normal ? ni_nt : ca1 * sa2;
+    float cam = ca1ca2 + sa1sa2, sap = sa1ca2 + ca1sa2, sam = sa1ca2 - ca1sa2;
+    float r = sam / (sap * cam);
+    r *= r;
+    r *= (ca1ca2 * ca1ca2 + sa1sa2 * sa1sa2);
+    if (ca1 < kCos90F || sa2 == 1.0f) r = 1.0f;
+    return r;
+}
+
+// SpinTheta, src/octmps_kernel.cuh:807-820 (hg_a = 1-g^2, hg_b = 1/(2g)).
+__device__ __forceinline__ float hg_cos(float g, float hg_a, float hg_b, float u_oc) {
+    if (g == 0.0f) return 2.0f * u_oc - 1.0f;
+    float t = hg_a / (1.0f - g + 2.0f * g * u_oc);
+    float c = (1.0f + g * g - t * t) * hg_b;
+    return fminf(fmaxf(c, -1.0f), 1.0f);
+}
+
+// SpinThetaForwardFstBias, src/octmps_kernel.cuh:881-896.
+__device__ __forceinline__ float bias_cos(float a, float inv_1ma, float inv_sq, float inv_2a,
+                                          float opa2, float u_oc) {
+    if (a == 0.0f) return u_oc;
+    float t = u_oc * inv_1ma + (1.0f - u_oc) * inv_sq;
+    float c = (opa2 - 1.0f / (t * t)) * inv_2a;
+    return fminf(fmaxf(c, -1.0f), 1.0f);
+}
+
+// Rotation of Spin / SpinBias (src/octmps_kernel.cuh:850-866, 1009-1018) around axis
+// (ax, ay, az), written with sqrt(ax^2+ay^2) in place of sqrt(1-az^2) and renormalised
+// like the reference's SINGLE_PRECISION variant (src/octmps_kernel.cuh:870-877).
+// copysign_mode: 0 -> uz = cost*sign(az) (Spin), 1 -> copysign(cost, az*cost) (SpinBias).
+__device__ __forceinline__ void rotate(float ax, float ay, float az, float cost, float r_psi,
+                                       int copysign_mode, float &ux, float &uy, float &uz) {
+    float sint = sqrtf(fmaxf(1.0f - cost * cost, 0.0f));
+    float sinp, cosp;
+    sincospif(2.0f * r_psi, &sinp, &cosp);
+    float stcp = sint * cosp, stsp = sint * sinp;
+    float sp2 = ax * ax + ay * ay;
+    float nx, ny, nz;
+    if (sp2 <= kNormalSin2) {
+        nx = stcp;
+        ny = stsp;
+        nz = copysign_mode ? copysignf(cost, az * cost) : cost * (az >= 0.0f ? 1.0f : -1.0f);
+    } else {
+        float rs = rsqrtf(sp2);
+        nx = (stcp * ax * az - stsp * ay) * rs + ax * cost;
+        ny = (stcp * ay * az + stsp * ax) * rs + ay * cost;
+        nz = -stcp * (sp2 * rs) + az * cost;
+    }
+    float rn = rsqrtf(nx * nx + ny * ny + nz * nz);
+    ux = nx * rn;
+    uy = ny * rn;
+    uz = nz * rn;
+}
+
+// Exact Q64.64 fixed point of a non-negative float (truncated below 2^-64).
+__device__ __forceinline__ void to_q64(float c, uint64_t &lo, uint64_t &hi) {
+    lo = 0;
+    hi = 0;
+    if (!(c > 0.0f) || !(c < 1.8e19f)) return;
+    uint32_t b = __float_as_uint(c);
+    int e = (int)((b >> 23) & 0xffu);
+    uint64_t m = b & 0x7fffffu;
+    if (e == 0) e = 1;
+    else m |= 0x800000u;
+    int sh = e - 86;  // c * 2^64 = m * 2^(e - 150 + 64)
+    if (sh <= -24) return;
+    if (sh < 0) {
+        lo = m >> (-sh);
+    } else if (sh < 64) {
+        lo = m << sh;
+        hi = (sh > 40) ? (m >> (64 - sh)) : 0;
+    } else {
+        hi = m << (sh - 64);
+    }
+}
+
+__device__ __forceinline__ void add_q64(uint64_t *lo_p, uint64_t *hi_p, uint64_t lo, uint64_t hi) {
+    if (lo) {
+        uint64_t old = atomicAdd((unsigned long long *)lo_p, (unsigned long long)lo);
+        if (old + lo < old) hi += 1;
+    }
+    if (hi) atomicAdd((unsigned long long *)hi_p, (unsigned long long)hi);
+}
+
+// Detector tally, src/octmps_kernel.cuh:684-741: six bins around the optical-depth bin,
+// Class I when MaxDepth > OP/2 - Lc/2.
+__device__ void tally_detection(const KArgs &A, const DevSet &S, uint32_t slot, float op, float maxz,
+                                float c) {
+    uint64_t *T = A.tallies + A.slots[slot].tally_off;
+    atomicAdd((unsigned long long *)&T[0], 1ull);
+    float depth = op * 0.5f;
+    float q = (depth - S.shift) / S.bin_w;
+    int vp = q <= 0.0f ? 0 : (q >= 1.0e9f ? 1000000000 : (int)q);
+    bool c1 = maxz > depth - S.half_lc;
+    int cls = c1 ? 0 : 1;
+    int B = S.num_bins;
+    uint64_t lo, hi, slo, shi;
+    to_q64(c, lo, hi);
+    to_q64(c * c, slo, shi);
+    uint64_t maxbits = (uint64_t)__double_as_longlong((double)c);
+    uint64_t *base = T + 4 + (uint64_t)cls * 6 * (uint64_t)B;
+    for (int ii = vp - 3; ii < vp + 3; ++ii) {
+        if (ii >= 0 && ii < B) {
+            if (ii == vp) atomicAdd((unsigned long long *)&T[1 + cls], 1ull);
+            add_q64(&base[ii], &base[B + ii], lo, hi);
+            add_q64(&base[2 * B + ii], &base[3 * B + ii], slo, shi);
+            atomicAdd((unsigned long long *)&base[4 * B + ii], 1ull);
+            atomicMax((unsigned long long *)&base[5 * B + ii], (unsigned long long)maxbits);
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+    uint32_t lo = __shfl((unsigned int)(v & 0xffffffffull), src);
+    uint32_t hi = __shfl((unsigned int)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <bool MULTI, bool LOG>
+__global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
+    const uint32_t lane = __lane_id();
+    const uint64_t gl = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+
+    // Photon (PhotonStructGPU, src/octmps_kernel.h:219-255) with sleft kept as tau.
+    float x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 1, w = 0, tau = 0, op = 0, maxz = 0;
+    float lr = 1, lr_fst = 1, loc_fst = -1;
+    int tet = 0, region = 1;
+    uint32_t flags = 0, steps = 0;
+    // Continuation photon (photon_cont): the fields the restore can change.
+    float cx = 0, cy = 0, cz = 0, cux = 0, cuy = 0, cuz = 1, cw = 0, cop = 0, cmaxz = 0;
+    int ctet = 0, cregion = 1;
+    // Streams: General, ReflectT, Spin.
+    uint64_t sG = 1, sR = 1, sS = 1;
+    uint32_t aG = 2, aR = 2, aS = 2;
+    // A-scan of the photon.
+    uint32_t slot = 0;
+    uint64_t id = 0;
+    float px = 0, py = 0, pz = 0;
+    int set = 0;
+    // Current region constants kept in registers.
+    float rn = 1, rmuas = 1, muas = 1;
+    uint32_t ndet = 0;
+    float csum = 0;
+
+    auto load_region = [&](int r) {
+        const DevRegion &R = A.regions[A.sets[set].region_base + r];
+        rn = R.n;
+        rmuas = R.rmuas;
+        muas = R.muas;
+    };
+
+    if (A.restore) {
+        const SavedLane &s = A.save[gl];
+        if (s.flags & F_ALIVE) {
+            x = s.x; y = s.y; z = s.z; ux = s.ux; uy = s.uy; uz = s.uz; w = s.w; tau = s.tau;
+            op = s.op; maxz = s.maxz; lr = s.lr; lr_fst = s.lr_fst; loc_fst = s.loc_fst;
+            tet = s.tet; region = s.region; flags = s.flags; steps = s.steps;
+            aG = s.aG; aR = s.aR; aS = s.aS; sG = s.sG; sR = s.sR; sS = s.sS;
+            cx = s.cx; cy = s.cy; cz = s.cz; cux = s.cux; cuy = s.cuy; cuz = s.cuz; cw = s.cw;
+            cop = s.cop; cmaxz = s.cmaxz; ctet = s.ctet; cregion = s.cregion;
+            slot = s.slot; id = s.id; ndet = s.ndet; csum = s.csum;
+            const DevScan &sc = A.slots[slot];
+            px = sc.px; py = sc.py; pz = sc.pz;
+            set = MULTI ? sc.set : 0;
+            load_region(region);
+        }
+    }
+
+    bool drained = false;
+    uint64_t wave_steps = 0, iters = 0;
+    const uint32_t q = A.refill_q < 1u ? 1u : A.refill_q;
+
+    for (;;) {
+        // ---- refill dead lanes (wave-uniform decision)
+        {
+            const bool alive = (flags & F_ALIVE) != 0;
+            const uint64_t dmask = __ballot(!alive);
+            const uint32_t ndead = (uint32_t)__popcll(dmask);
+            if (!drained && ndead != 0 && (ndead >= q || ndead == 64u)) {
+                const int leader = __ffsll((unsigned long long)dmask) - 1;
+                uint64_t base = 0;
+                if ((int)lane == leader)
+                    base = atomicAdd((unsigned long long *)&A.ctrl[CTRL_NEXT_ID], (unsigned long long)ndead);
+                base = shfl_u64(base, leader);
+                if (!alive) {
+                    const uint32_t rank = (uint32_t)__popcll(dmask & ((1ull << lane) - 1ull));
+                    const uint64_t nid = base + rank;
+                    if (nid < A.id_end) {
+                        // LaunchPhoton, src/octmps_kernel.cuh:475-503.
+                        int lo = 0, hi = A.n_seg - 1;
+                        while (lo < hi) {
+                            int mid = (lo + hi + 1) >> 1;
+                            if (A.seg_start[mid] <= nid) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        slot = A.seg_slot[lo];
+                        const DevScan &sc = A.slots[slot];
+                        const uint64_t pidx = sc.photon_offset + A.seg_poff[lo] + (nid - A.seg_start[lo]);
+                        const uint64_t h = photon_hash(sc.seed, sc.salt, pidx);
+                        const uint32_t k = (uint32_t)pidx & (A.n_triplets - 1u);
+                        aG = A.mult[3 + 3 * k];
+                        aR = A.mult[4 + 3 * k];
+                        aS = A.mult[5 + 3 * k];
+                        const uint32_t b0 = A.mult[0], b1 = A.mult[1], b2 = A.mult[2];
+                        sG = derive_state(seeding_state(h, 0, b0), b0, aG);
+                        sR = derive_state(seeding_state(h, 1, b1), b1, aR);
+                        sS = derive_state(seeding_state(h, 2, b2), b2, aS);
+                        px = sc.px; py = sc.py; pz = sc.pz;
+                        x = px; y = py; z = pz;
+                        ux = 0.0f; uy = 0.0f; uz = 1.0f;
+                        w = sc.w0;
+                        tau = 0.0f; op = 0.0f; maxz = 0.0f;
+                        lr = 1.0f; lr_fst = 1.0f; loc_fst = -1.0f;
+                        tet = sc.root_tet;
+                        region = sc.root_region;
+                        set = MULTI ? sc.set : 0;
+                        flags = F_ALIVE;
+                        steps = 0;
+                        id = nid;
+                        ndet = 0;
+                        csum = 0.0f;
+                        load_region(region);
+                    }
+                }
+                if (base + ndead >= A.id_end) drained = true;
+            }
+        }
+        const uint64_t amask = __ballot((flags & F_ALIVE) != 0);
+        if (amask == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (A.slice_mode && ((drained && A.exit_on_drain) || (A.iter_budget && iters >= A.iter_budget))) break;
+        ++iters;
+        wave_steps += (uint64_t)__popcll(amask);
+        if (!(flags & F_ALIVE)) continue;
+
+        const DevSet &S = A.sets[set];
+        bool finish = false, killed = false;
+        if (++steps > A.max_steps) {
+            finish = true;
+            killed = true;
+        } else {
+            // ---- ComputeStepSize, src/octmps_kernel.cuh:505-515
+            float tau_d = tau;
+            if (tau_d == 0.0f) tau_d = -logf(rand_oc(sG, aG));
+            float s = tau_d * rmuas;
+            tau = 0.0f;
+            // ---- HitBoundary, src/octmps_kernel.cuh:517-572
+            const float4 *P = A.planes + (size_t)tet * 4;
+            const float4 p0 = P[0], p1 = P[1], p2 = P[2], p3 = P[3];
+            const int4 L = *reinterpret_cast<const int4 *>(&A.topo[tet].link[0]);
+            const uint32_t nregs = A.topo[tet].nbr_regions;
+            const float c0 = p0.x * ux + p0.y * uy + p0.z * uz;
+            const float c1 = p1.x * ux + p1.y * uy + p1.z * uz;
+            const float c2 = p2.x * ux + p2.y * uy + p2.z * uz;
+            const float c3 = p3.x * ux + p3.y * uy + p3.z * uz;
+            float dmin = 1e10f, mincos = 0.0f;
+            int imin = -1;
+#define OCTMPS_FACE(I, PL, C)                                            \
+    if (C < 0.0f) {                                                     \
+        const float perp = PL.x * x + PL.y * y + PL.z * z + PL.w;       \
+        const float d = -perp / C;                                      \
+        if (d < dmin) {                                                 \
+            dmin = d;                                                   \
+            imin = I;                                                   \
+            mincos = C;                                                 \
+        }                                                               \
+    }
+            OCTMPS_FACE(0, p0, c0)
+            OCTMPS_FACE(1, p1, c1)
+            OCTMPS_FACE(2, p2, c2)
+            OCTMPS_FACE(3, p3, c3)
+#undef OCTMPS_FACE
+            const bool hit = imin >= 0 && s > dmin;
+            if (hit) {
+                // Glass (mus = 0) keeps the leftover unchanged, as (s - d) / DBL_MAX does
+                // in the reference; an overflowed step stays infinite.
+                tau = (muas == 0.0f) ? (isinf(s) ? INFINITY : tau_d) : (s - dmin) * muas;
+                s = dmin;
+            }
+            // ---- Hop, src/octmps_kernel.cuh:574-582
+            x = fmaf(s, ux, x);
+            y = fmaf(s, uy, y);
+            z = fmaf(s, uz, z);
+            op += s;
+            if (maxz < z) maxz = z;
+
+            if (hit) {
+                // ---- FastReflectTransmit, src/octmps_kernel.cuh:590-805
+                const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
+                const int nxt = imin == 0 ? L.x : (imin == 1 ? L.y : (imin == 2 ? L.z : L.w));
+                const int nr = (int)((nregs >> (8 * imin)) & 0xffu);
+                const float ca1 = -mincos;
+                float nt;
+                if (nxt < 0) nt = S.n_ambient;
+                else nt = (nr == region) ? rn : A.regions[S.region_base + nr].n;
+                const float rnd = rand_co(sR, aR);
+                float R;
+                if (nxt >= 0 && nt == rn) R = (ca1 < kCos90F) ? 1.0f : 0.0f;  // sam == 0
+                else R = fresnel_f(rn, nt, ca1);
+                const float two_ca1 = 2.0f * ca1;
+                const float rx = fmaf(two_ca1, pf.x, ux);
+                const float ry = fmaf(two_ca1, pf.y, uy);
+                const float rz = fmaf(two_ca1, pf.z, uz);
+                if (nxt < 0 && R < 1.0f) {
+                    if (uz < 0.0f) {
+                        bool flag = false;
+                        const float dx = x - px, dy = y - py;
+                        if (dx * dx + dy * dy < S.radius2 && -uz > S.cos_angle) {
+                            const int tb = S.type_bias;
+                            if (tb == 0 || tb == 3) flag = true;
+                            if (((flags & F_FST) && tb == 37) || (loc_fst == maxz && tb != 37) ||
+                                (flags & F_NBS))
+                                flag = true;
+                        }
+                        if (flag) {
+                            const float c = w * (1.0f - R) * lr;
+                            tally_detection(A, S, slot, op, maxz, c);
+                            if (LOG) {
+                                ++ndet;
+                                csum += c;
+                            }
+                        }
+                    } else {
+                        flags |= F_NBS;
+                    }
+                    w *= R;
+                    ux = rx;
+                    uy = ry;
+                    uz = rz;
+                } else if (rnd > R) {
+                    tet = nxt;
+                    if (nr != region) {
+                        region = nr;
+                        load_region(region);
+                    }
+                } else {
+                    if (uz > 0.0f) flags |= F_NBS;
+                    ux = rx;
+                    uy = ry;
+                    uz = rz;
+                }
+            } else {
+                // ---- Drop, src/octmps_kernel.cuh:584-588
+                const DevRegion RG = A.regions[S.region_base + region];
+                w -= w * RG.mua_muas;
+                const int tb = S.type_bias;
+                if (tb == 0) {
+                    // Spin, src/octmps_kernel.cuh:822-879
+                    const float cost = hg_cos(RG.g, RG.hg_a, RG.hg_b, rand_oc(sS, aS));
+                    rotate(ux, uy, uz, cost, rand_co(sS, aS), 0, ux, uy, uz);
+                } else if (tb == 37) {
+                    // SpinBias, src/octmps_kernel.cuh:898-1086
+                    const float ux0 = ux, uy0 = uy, uz0 = uz;
+                    int F = 0;
+                    bool first = false;
+                    float atmp = 0.0f;
+                    if (z > S.td_min && z < S.td_max && uz > 0.0f && !(flags & F_FST)) {
+                        cx = x; cy = y; cz = z; cux = ux; cuy = uy; cuz = uz; cw = w;
+                        cop = op; cmaxz = maxz; ctet = tet; cregion = region;
+                        flags = (flags & ~F_CNBS) | ((flags & F_NBS) ? F_CNBS : 0u);
+                        F = 1;
+                        flags |= F_FST;
+                        loc_fst = z;
+                        atmp = S.bias_a;
+                        first = true;
+                    }
+                    float vx = px - x, vy = py - y, vz = pz - z;
+                    const float len = sqrtf(vx * vx + vy * vy + vz * vz);
+                    vx = vx / len;
+                    vy = vy / len;
+                    vz = vz / len;
+                    if (((flags & F_FST) || (flags & F_NBS)) && F == 0) {
+                        F = 2;
+                        const float next = S.next_step_num / muas;
+                        const float dxp = x - px, dyp = y - py, dzp = z - pz;
+                        const float dist = sqrtf(dxp * dxp + dyp * dyp + dzp * dzp);
+                        if (next >= dist && -vz >= S.cos_angle) F = 1;
+                        atmp = S.bias_a;
+                    }
+                    int sel = 0;
+                    float cost, fx = ux, fy = uy, fz = uz;
+                    if (F) {
+                        sel = (rand_co(sS, aS) <= S.p_add) ? 1 : 2;
+                        if (F == 1 || sel == 1) {
+                            cost = bias_cos(atmp, S.inv_1ma, S.inv_sq, S.inv_2a, S.opa2, rand_oc(sS, aS));
+                            fx = vx;
+                            fy = vy;
+                            fz = vz;
+                        } else {
+                            cost = hg_cos(RG.g, RG.hg_a, RG.hg_b, rand_oc(sS, aS));
+                        }
+                    } else {
+                        cost = hg_cos(RG.g, RG.hg_a, RG.hg_b, rand_oc(sS, aS));
+                    }
+                    cost = fminf(fmaxf(cost, -1.0f), 1.0f);
+                    rotate(fx, fy, fz, cost, rand_co(sS, aS), 1, ux, uy, uz);
+                    if (atmp != 0.0f) {
+                        const float costg = ux0 * ux + uy0 * uy + uz0 * uz;
+                        const float g = RG.g;
+                        float factor;
+                        if (F == 1) {
+                            // Equation (8), src/octmps_kernel.cuh:1047-1050.
+                            const float t1 = S.opa2 - 2.0f * atmp * cost;
+                            const float t2 = 1.0f + g * g - 2.0f * g * costg;
+                            factor = RG.lr8 * (t1 * sqrtf(t1)) / (t2 * sqrtf(t2));
+                        } else {
+                            // Equation (9), src/octmps_kernel.cuh:1071-1078.
+                            const float costg1 = vx * ux + vy * uy + vz * uz;
+                            const float k1 = sel == 1 ? cost : costg1;
+                            const float k2 = sel == 1 ? costg : cost;
+                            const float t1 = S.opa2 - 2.0f * atmp * k1;
+                            const float pdf1 = S.pdfb_k / (t1 * sqrtf(t1));
+                            const float t2 = sqrtf(1.0f + g * g - 2.0f * g * k2);
+                            const float pdf2 = RG.hg_a / (2.0f * (t2 * t2 * t2));
+                            factor = pdf2 / (S.p_add * pdf1 + (1.0f - S.p_add) * pdf2);
+                        }
+                        lr *= factor;
+                        if (first) lr_fst = lr;
+                    }
+                }
+            }
+            // ---- Roulette, src/octmps_kernel.cuh:1376-1392
+            if (w < kWeightF) {
+                const float r = rand_co(sG, aG);
+                if (w != 0.0f && r < kChanceF) {
+                    w *= 10.0f;
+                } else if ((flags & F_FST) && S.type_bias != 3) {
+                    const float lrt = lr_fst;
+                    x = cx; y = cy; z = cz; ux = cux; uy = cuy; uz = cuz; w = cw;
+                    op = cop; maxz = cmaxz; tet = ctet;
+                    tau = 0.0f;
+                    flags = F_ALIVE | ((flags & F_CNBS) ? F_NBS : 0u);
+                    loc_fst = -1.0f;
+                    if (cregion != region) {
+                        region = cregion;
+                        load_region(region);
+                    }
+                    const DevRegion &RC = A.regions[S.region_base + region];
+                    const float cost = hg_cos(RC.g, RC.hg_a, RC.hg_b, rand_oc(sS, aS));
+                    rotate(ux, uy, uz, cost, rand_co(sS, aS), 0, ux, uy, uz);
+                    lr = (lrt < 1.0f) ? 1.0f - lrt : 1.0f;
+                } else {
+                    finish = true;
+                }
+            }
+        }
+        if (finish) {
+            flags = 0;
+            atomicAdd((unsigned long long *)&A.slot_done[slot], 1ull);
+            if (killed) atomicAdd((unsigned long long *)&A.ctrl[CTRL_KILLED], 1ull);
+            if (LOG) {
+                PhotonLogRec r;
+                r.steps = steps;
+                r.detections = ndet;
+                r.contrib = csum;
+                r.killed = killed ? 1u : 0u;
+                A.plog[id] = r;
+            }
+        }
+    }
+
+    if (A.slice_mode) {
+        SavedLane &s = A.save[gl];
+        s.flags = flags;
+        if (flags & F_ALIVE) {
+            s.x = x; s.y = y; s.z = z; s.ux = ux; s.uy = uy; s.uz = uz; s.w = w; s.tau = tau;
+            s.op = op; s.maxz = maxz; s.lr = lr; s.lr_fst = lr_fst; s.loc_fst = loc_fst;
+            s.tet = tet; s.region = region; s.steps = steps;
+            s.aG = aG; s.aR = aR; s.aS = aS; s.sG = sG; s.sR = sR; s.sS = sS;
+            s.cx = cx; s.cy = cy; s.cz = cz; s.cux = cux; s.cuy = cuy; s.cuz = cuz; s.cw = cw;
+            s.cop = cop; s.cmaxz = cmaxz; s.ctet = ctet; s.cregion = cregion;
+            s.slot = slot; s.id = id; s.ndet = ndet; s.csum = csum;
+        }
+    }
+    if (lane == 0 && wave_steps) atomicAdd((unsigned long long *)&A.ctrl[CTRL_STEPS], (unsigned long long)wave_steps);
+}
+
+__global__ void fill_u64_kernel(uint64_t *p, uint64_t n, uint64_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void clear_saved_kernel(SavedLane *p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i].flags = 0;
+}
+
+__global__ void probe_rng_kernel(const uint32_t *mult, uint32_t K, uint64_t seed, uint64_t salt,
+                                 uint64_t photon0, int n_photons, int n_draws, float *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_photons) return;
+    uint64_t pidx = photon0 + (uint64_t)i;
+    uint64_t h = photon_hash(seed, salt, pidx);
+    uint32_t k = (uint32_t)pidx & (K - 1u);
+    for (int s = 0; s < 3; ++s) {
+        uint32_t b = mult[s], a = mult[3 + 3 * k + s];
+        uint64_t st = derive_state(seeding_state(h, (uint32_t)s, b), b, a);
+        for (int d = 0; d < n_draws; ++d) out[((size_t)i * 3 + s) * n_draws + d] = rand_co(st, a);
+    }
+}
+
+__global__ void probe_fresnel_kernel(int n, const float *ni, const float *nt, const float *ca1, float *r) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = fresnel_f(ni[i], nt[i], ca1[i]);
+}
+
+__global__ void probe_sampler_kernel(int kind, float p, int n, const float *u, float *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (kind == 0) {
+        out[i] = hg_cos(p, 1.0f - p * p, p != 0.0f ? 1.0f / (2.0f * p) : 0.0f, u[i]);
+    } else {
+        float a = p;
+        float sq = sqrtf(1.0f + a * a);
+        out[i] = bias_cos(a, a != 1.0f ? 1.0f / (1.0f - a) : 0.0f, 1.0f / sq, a != 0.0f ? 1.0f / (2.0f * a) : 0.0f,
+                          1.0f + a * a, u[i]);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_transport(const KArgs &a, int grid, int block, bool multi, bool log, hipStream_t stream) {
+    if (multi) {
+        if (log) hipLaunchKernelGGL((transport_kernel<true, true>), dim3(grid), dim3(block), 0, stream, a);
+        else hipLaunchKernelGGL((transport_kernel<true, false>), dim3(grid), dim3(block), 0, stream, a);
+    } else {
+        if (log) hipLaunchKernelGGL((transport_kernel<false, true>), dim3(grid), dim3(block), 0, stream, a);
+        else hipLaunchKernelGGL((transport_kernel<false, false>), dim3(grid), dim3(block), 0, stream, a);
+    }
+    return hipGetLastError();
+}
+
+int transport_blocks_per_cu(int block, bool multi, bool log) {
+    int n = 0;
+    hipError_t e;
+    if (multi) {
+        e = log ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<true, true>, block, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<true, false>, block, 0);
+    } else {
+        e = log ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<false, true>, block, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<false, false>, block, 0);
+    }
+    return e == hipSuccess ? n : 0;
+}
+
+hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_saved(SavedLane *p, uint64_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(clear_saved_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_rng(const uint32_t *mult, uint32_t K, uint64_t seed, uint64_t salt, uint64_t photon0,
+                            int n_photons, int n_draws, float *out, hipStream_t stream) {
+    hipLaunchKernelGGL(probe_rng_kernel, dim3((n_photons + 63) / 64), dim3(64), 0, stream, mult, K, seed, salt,
+                       photon0, n_photons, n_draws, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_fresnel(int n, const float *ni, const float *nt, const float *ca1, float *r,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(probe_fresnel_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, ni, nt, ca1, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_sampler(int kind, float p, int n, const float *u, float *out, hipStream_t stream) {
+    hipLaunchKernelGGL(probe_sampler_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, kind, p, n, u, out);
+    return hipGetLastError();
+}
+
+}  // namespace octmps

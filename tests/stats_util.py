@@ -1,0 +1,34 @@
+"""Statistical comparison of two tallies (GPU vs oracle), shared by the parity tests."""
+import numpy as np
+
+
+def bin_stats(total, total_sq, n):
+    """Mean and standard error per bin with the reference's estimator
+    (WriteReflectanceClassI_and_ClassII, src/octmps_io.cpp:309-318)."""
+    mean = total / n
+    var = (total_sq / n - mean * mean) / (n - 1)
+    return mean, np.sqrt(np.maximum(var, 0.0))
+
+
+def compare(gpu, ref, n):
+    """Per-bin z scores over both classes, plus total reflectance of each side.
+
+    Acceptance used by the tests: <= 1% of populated bins beyond 3 sigma_comb, none beyond
+    5 sigma_comb, and total R (sum over bins of Class I + Class II means) within 0.5%."""
+    mg, sg = bin_stats(gpu["sum"], gpu["sumsq"], n)
+    mr, sr = bin_stats(ref["sum"], ref["sumsq"], n)
+    sig = np.sqrt(sg ** 2 + sr ** 2)
+    pop = sig > 0
+    z = np.zeros_like(mg)
+    z[pop] = (mg[pop] - mr[pop]) / sig[pop]
+    same_zero = (~pop) & (mg == mr)
+    rg, rr = mg.sum(), mr.sum()
+    return {
+        "bins": int(pop.sum()),
+        "frac_3sigma": float((np.abs(z[pop]) > 3).mean()) if pop.any() else 0.0,
+        "max_abs_z": float(np.abs(z[pop]).max()) if pop.any() else 0.0,
+        "unpopulated_mismatch": int((~pop & ~same_zero).sum()),
+        "R_gpu": float(rg),
+        "R_ref": float(rr),
+        "R_rel": float(abs(rg - rr) / rr) if rr else float(abs(rg - rr)),
+    }

@@ -1,0 +1,121 @@
+"""GPU parity of the transport path against the FP64 CPU oracle (device 0)."""
+import numpy as np
+import pytest
+
+import octmps as O
+import oracle_py as OR
+from octmps import configs
+from stats_util import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def test_rng_streams_bit_exact(ctx, table):
+    # Per-photon stream seeding and MWC draws of the device equal the oracle's bit for bit.
+    seed, salt, p0, n, d = 1, 5, 1000, 64, 16
+    dev = ctx.probe_rng(seed, salt, p0, n, d)
+    for i in range(n):
+        x, a = OR.photon_streams(seed, salt, p0 + i, table)
+        for s in range(3):
+            ref, _ = OR.rand_co(int(x[s]), int(a[s]), d)
+            assert np.array_equal(dev[i, s].astype(np.float64), ref), (i, s)
+
+
+def test_fresnel_device_vs_oracle(ctx):
+    rng = np.random.default_rng(3)
+    ni = rng.uniform(1.0, 1.6, 4000).astype(np.float32)
+    nt = rng.uniform(1.0, 1.6, 4000).astype(np.float32)
+    ca1 = rng.uniform(0.0, 1.0, 4000).astype(np.float32)
+    ca1[:50] = 1.0
+    r = ctx.probe_fresnel(ni, nt, ca1)
+    ref = np.array([OR.fresnel(float(a), float(b), float(c)) for a, b, c in zip(ni, nt, ca1)])
+    # Away from the critical angle, where R jumps to 1, FP32 agrees to ~1e-5 absolute.
+    sa1 = np.sqrt(np.maximum(1 - ca1.astype(np.float64) ** 2, 0))
+    crit = np.abs(ni.astype(np.float64) / nt * sa1 - 1.0) < 1e-3
+    assert np.max(np.abs(r[~crit] - ref[~crit])) < 2e-5
+
+
+@pytest.mark.parametrize("kind,param", [(0, 0.9), (0, 0.0), (0, 0.95), (1, 0.925), (1, 0.5)])
+def test_samplers_device_vs_oracle(ctx, kind, param):
+    u = (np.arange(1, 4097, dtype=np.float64) / 4096.0).astype(np.float32)
+    dev = ctx.probe_sampler(kind, param, u)
+    f = OR.spin_theta if kind == 0 else OR.spin_theta_bias
+    ref = np.array([f(param, float(v)) for v in u])
+    assert np.max(np.abs(dev - ref)) < 5e-5
+
+
+def _run_pair(ctx, table, mesh, params, scan, seed, threads=8):
+    mid = ctx.upload(mesh)
+    t, st, plog = ctx.run_bscan(mid, params, [scan], seed, log=True)
+    gpu = O.decode_tally(t[0], params.num_depth_bins)
+    ref = OR.run_ascan(mesh.soa(), params, scan, seed, table, threads=threads, log=True)
+    return gpu, st, plog, ref
+
+
+def test_sphere_parity_paired(ctx, table):
+    n = 20000
+    mesh, params, scans, seed = configs.sphere(cells=16, photons=n)
+    gpu, st, plog, ref = _run_pair(ctx, table, mesh, params, scans[0], seed)
+    assert st.photons == n and st.killed == 0 and ref["killed"] == 0
+    print("steps gpu/ref", st.photon_steps, ref["steps"])
+    same_steps = np.mean(plog["steps"].astype(np.float64) == ref["log"][:, 0])
+    same_det = np.mean(plog["detections"].astype(np.float64) == ref["log"][:, 1])
+    print("paired photons: same steps", same_steps, "same detections", same_det)
+    c = compare(gpu, ref, n)
+    print(c)
+    assert same_steps > 0.9
+    assert c["frac_3sigma"] <= 0.01 and c["max_abs_z"] < 5
+    assert c["R_rel"] < 0.005
+
+
+def test_glass_slab_known_answer(ctx):
+    # One glass region (mus = 0: no scattering) with n = 1.5 under air. Every photon travels
+    # straight down and up; at the first return it is detected with weight
+    # c1 = (1-R)^2 R, R = ((1.5-1)/(1.5+1))^2, at optical depth d (Class I, six bins).
+    d, h, cells = 0.1, 0.1 / 8, 8
+    mesh = O.Mesh.generate(O.GEOM_HOMOGENEOUS, (cells, cells, cells), (cells * h, cells * h, d))
+    params = O.make_params([(1.5, 0.0, 0.0, 0.9)], num_bins=900, depth=0.27)
+    x0, y0 = 0.13 * h, 0.07 * h
+    root, z = mesh.find_root(x0, y0, 0.0)
+    n = 4000
+    scan = O.ascan(x0, y0, z, root, n)
+    mid = ctx.upload(mesh)
+    t, st = ctx.run_ascan(mid, params, scan, 1)
+    g = O.decode_tally(t, params.num_depth_bins)
+    R = ((1.5 - 1.0) / 2.5) ** 2
+    c1 = (1 - R) ** 2 * R
+    w = params.coherence_length_source / 6
+    vp = int(np.floor(d / w))
+    assert g["num_class1"] == n
+    cls1 = g["sum"][0] / n
+    for b in range(vp - 3, vp + 3):
+        assert g["count"][0][b] == n
+        assert abs(cls1[b] - c1) < 1e-6 * c1
+    # Second return (Class II): expected (1-R)^2 R^3 per photon after unbiased roulette.
+    vp2 = int(np.floor(2 * d / w))
+    m2 = g["sum"][1][vp2] / n
+    e2 = (1 - R) ** 2 * R ** 3
+    se = np.sqrt((g["sumsq"][1][vp2] / n - m2 * m2) / (n - 1))
+    assert abs(m2 - e2) < 5 * se + 1e-12
+
+
+def test_batched_equals_plain_bitwise(ctx):
+    mesh, params, scans, seed = configs.slab4(cells=(16, 16, 8), n_scans=3, photons=3000, bins=128)
+    mid = ctx.upload(mesh)
+    tb, stb = ctx.run_bscan(mid, params, scans, seed)
+    for i, s in enumerate(scans):
+        ta, sta = ctx.run_ascan(mid, params, s, seed)
+        assert np.array_equal(ta, tb[i]), i
+
+
+def test_graph_replay_matches_eager(ctx):
+    mesh, params, scans, seed = configs.slab4(cells=(16, 16, 8), n_scans=4, photons=2000, bins=128)
+    mid = ctx.upload(mesh)
+    te, _ = ctx.run_bscan(mid, params, scans, seed)
+    o = O.BatchOpts()
+    o.use_graph = 1
+    for _ in range(3):
+        tg, _ = ctx.run_bscan(mid, params, scans, seed, opts=o)
+        assert np.array_equal(te, tg)
+        ta, _ = ctx.run_ascan(mid, params, scans[0], seed)  # eager work between replays
+        assert np.array_equal(ta, te[0])
