@@ -1,0 +1,45 @@
+"""The C-ABI library loads without a GPU and exports every symbol include/octmps.h declares."""
+import ctypes
+import os
+import re
+
+import octmps as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "octmps.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(octmps_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) > 40
+    lib = ctypes.CDLL(O.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    bound = {n for n, _, _ in O.SIGNATURES}
+    assert set(declared_functions()) == bound
+
+
+def test_status_strings_and_version():
+    assert O.lib.octmps_status_string(0) == b"ok"
+    assert O.lib.octmps_status_string(4).startswith(b"probe")
+    assert b"gfx950" in O.lib.octmps_version()
+
+
+def test_no_device_is_an_error_not_a_crash():
+    # Without a GPU the context refuses loudly; there is no CPU fallback.
+    if os.path.exists("/dev/kfd"):
+        return
+    try:
+        O.Context(0)
+    except O.OctmpsError as e:
+        assert e.status == 8
+    else:
+        raise AssertionError("context created without a device")
