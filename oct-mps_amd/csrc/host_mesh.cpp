@@ -69,66 +69,103 @@ octmps_status build_topology(HostMesh &m) {
             return fail(OCTMPS_E_RANGE, "tet region must be in 1..99");
         m.max_region = std::max(m.max_region, m.region[t]);
     }
+    // Faces are matched by vertex coordinates, like CheckFaceExists: vertices with equal
+    // coordinates (distinct ids do occur in sample meshes) count as one.
+    std::vector<int32_t> canon((size_t)V);
+    {
+        std::vector<int32_t> ord((size_t)V);
+        std::iota(ord.begin(), ord.end(), 0);
+        auto key = [&](int32_t v, int q) { return m.xyz[3 * (int64_t)v + q] + 0.0; };  // -0 == +0
+        std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+            for (int q = 0; q < 3; ++q)
+                if (key(a, q) != key(b, q)) return key(a, q) < key(b, q);
+            return a < b;
+        });
+        for (size_t i = 0; i < ord.size(); ++i) {
+            bool same = i > 0;
+            for (int q = 0; q < 3 && same; ++q) same = key(ord[i], q) == key(ord[i - 1], q);
+            canon[ord[i]] = same ? canon[ord[i - 1]] : ord[i];
+        }
+    }
     std::vector<FaceKey> keys((size_t)(4 * T));
     for (int64_t t = 0; t < T; ++t) {
         for (int j = 0; j < 4; ++j) {
             FaceKey &k = keys[(size_t)(4 * t + j)];
-            for (int q = 0; q < 3; ++q) k.v[q] = m.tv[4 * t + kFaceVerts[j][q]];
+            for (int q = 0; q < 3; ++q) k.v[q] = canon[m.tv[4 * t + kFaceVerts[j][q]]];
             std::sort(k.v, k.v + 3);
             k.tet = (int32_t)t;
             k.face = j;
         }
     }
-    std::sort(keys.begin(), keys.end());
+    std::sort(keys.begin(), keys.end(), [](const FaceKey &a, const FaceKey &b) {
+        if (a < b) return true;
+        if (b < a) return false;
+        return a.face < b.face;
+    });
     m.links.assign((size_t)(4 * T), -1);
     m.planes.assign((size_t)(16 * T), 0.0);
     std::vector<double> mids((size_t)(3 * T));
     for (int64_t t = 0; t < T; ++t) midpoint(m, t, mids[3 * t], mids[3 * t + 1], mids[3 * t + 2]);
 
-    size_t i = 0;
-    while (i < keys.size()) {
-        size_t j = i + 1;
-        while (j < keys.size() && keys[j].same(keys[i])) ++j;
-        if (j - i > 2) return fail(OCTMPS_E_PARSE, "a face is shared by more than two tetrahedra");
-        const FaceKey &c = keys[i];  // creator: lowest tet index
-        const int32_t *tv = &m.tv[4 * (int64_t)c.tet];
-        const double *V0 = &m.xyz[3 * tv[kFaceVerts[c.face][0]]];
-        const double *V1 = &m.xyz[3 * tv[kFaceVerts[c.face][1]]];
-        const double *V2 = &m.xyz[3 * tv[kFaceVerts[c.face][2]]];
+    // Plane of face (tet, j) from that tet's vertex order, turned inward for that tet. A
+    // zero-area face yields NaN like the reference's division by a zero norm.
+    auto face_plane = [&](int32_t tet, int j, double out[4]) {
+        const int32_t *tv = &m.tv[4 * (int64_t)tet];
+        const double *V0 = &m.xyz[3 * tv[kFaceVerts[j][0]]];
+        const double *V1 = &m.xyz[3 * tv[kFaceVerts[j][1]]];
+        const double *V2 = &m.xyz[3 * tv[kFaceVerts[j][2]]];
         long double x1 = V0[0] - V1[0], y1 = V0[1] - V1[1], z1 = V0[2] - V1[2];
         long double x2 = V0[0] - V2[0], y2 = V0[1] - V2[1], z2 = V0[2] - V2[2];
         long double Nx = y1 * z2 - z1 * y2;
         long double Ny = z1 * x2 - x1 * z2;
         long double Nz = x1 * y2 - y1 * x2;
         double norm = (double)sqrtl(Nx * Nx + Ny * Ny + Nz * Nz);
-        if (!(norm > 0)) return fail(OCTMPS_E_PARSE, "degenerate tetrahedron face");
         Nx /= norm;
         Ny /= norm;
         Nz /= norm;
         double nx = (double)Nx, ny = (double)Ny, nz = (double)Nz;
         double mnx = (double)(-Nx), mny = (double)(-Ny), mnz = (double)(-Nz);
         double d = mnx * V0[0] + mny * V0[1] + mnz * V0[2];
-        {   // orient inward for the creator
-            const double *mc = &mids[3 * (int64_t)c.tet];
-            long double h = nx * mc[0] + ny * mc[1] + nz * mc[2] + d;
-            if (h < 0) {
-                nx *= -1;
-                ny *= -1;
-                nz *= -1;
-                d *= -1;
-            }
+        const double *mc = &mids[3 * (int64_t)tet];
+        long double h = nx * mc[0] + ny * mc[1] + nz * mc[2] + d;
+        if (h < 0) {
+            nx *= -1;
+            ny *= -1;
+            nz *= -1;
+            d *= -1;
         }
+        out[0] = nx;
+        out[1] = ny;
+        out[2] = nz;
+        out[3] = d;
+    };
+
+    size_t i = 0;
+    while (i < keys.size()) {
+        size_t j = i + 1;
+        while (j < keys.size() && keys[j].same(keys[i])) ++j;
+        // Entries i..j-1 share one face, ordered by (tet, face). The first tet creates it
+        // (each of its own entries creates a face; later tets reuse the first one). Later
+        // tets link to the first tet; the first tet links to the last one
+        // (src/mesh.cpp:198-208).
+        const int32_t first = keys[i].tet, last = keys[j - 1].tet;
+        double base[4];
+        face_plane(first, keys[i].face, base);
         for (size_t k = i; k < j; ++k) {
             const FaceKey &e = keys[k];
-            const double *mt = &mids[3 * (int64_t)e.tet];
-            long double h = nx * mt[0] + ny * mt[1] + nz * mt[2] + d;
-            double s = (h < 0) ? -1.0 : 1.0;
             double *pl = &m.planes[16 * (int64_t)e.tet + 4 * e.face];
-            pl[0] = nx * s;
-            pl[1] = ny * s;
-            pl[2] = nz * s;
-            pl[3] = d * s;
-            if (j - i == 2) m.links[4 * (int64_t)e.tet + e.face] = keys[i + (k == i ? 1 : 0)].tet;
+            double f[4];
+            if (e.tet == first) face_plane(first, e.face, f);
+            else std::copy(base, base + 4, f);
+            const double *mt = &mids[3 * (int64_t)e.tet];
+            long double h = f[0] * mt[0] + f[1] * mt[1] + f[2] * mt[2] + f[3];
+            double s = (h < 0) ? -1.0 : 1.0;
+            pl[0] = f[0] * s;
+            pl[1] = f[1] * s;
+            pl[2] = f[2] * s;
+            pl[3] = f[3] * s;
+            int32_t link = (e.tet == first) ? (last != first ? last : -1) : first;
+            m.links[4 * (int64_t)e.tet + e.face] = link;
         }
         i = j;
     }
