@@ -1,0 +1,162 @@
+"""GPU tests of the batched and served paths, the CLI and whole-path properties."""
+import os
+import subprocess
+import time
+
+import numpy as np
+import pytest
+
+import octmps as O
+from octmps import configs
+from stats_util import bin_stats
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_meshes(ctx):
+    out = {}
+    m, p, s, seed = configs.slab4(cells=(16, 16, 8), n_scans=4, photons=1500, bins=128)
+    out["slab"] = (m, ctx.upload(m), p, s, seed)
+    m, p, s, seed = configs.sphere(cells=12, photons=1200, bins=96)
+    out["sphere"] = (m, ctx.upload(m), p, s, seed)
+    m, p, s, seed = configs.ellipsoid(cells=(16, 16, 8), n_scans=6, budgets=[300, 2000, 700, 50, 1200, 900],
+                                      bins=128)
+    out["ellipsoid"] = (m, ctx.upload(m), p, s, seed)
+    return out
+
+
+def test_uneven_budgets_batched_equals_plain(ctx, small_meshes):
+    m, mid, p, scans, seed = small_meshes["ellipsoid"]
+    tb, st = ctx.run_bscan(mid, p, scans, seed)
+    assert st.photons == sum(s.n_photons for s in scans) and st.killed == 0
+    for i, s in enumerate(scans):
+        ta, _ = ctx.run_ascan(mid, p, s, seed)
+        assert np.array_equal(ta, tb[i])
+
+
+def test_served_results_equal_batched_alone_and_in_a_mix(ctx, small_meshes):
+    ref = {k: ctx.run_bscan(v[1], v[2], v[3], v[4])[0] for k, v in small_meshes.items()}
+    srv = O.Server(ctx, slice_photons=4096)
+    try:
+        # alone
+        _, mid, p, s, seed = small_meshes["slab"]
+        info, out = srv.wait(srv.submit(mid, p, s, seed), 120)
+        assert info.status == 0 and np.array_equal(out, ref["slab"])
+        # in a mix of meshes, parameter sets and budgets, submitted together
+        ids = {k: srv.submit(v[1], v[2], v[3], v[4]) for k, v in small_meshes.items()}
+        for k, rid in ids.items():
+            info, out = srv.wait(rid, 120)
+            assert info.status == 0 and info.done
+            assert np.array_equal(out, ref[k]), k
+            assert info.t_submit_ms <= info.t_admit_ms <= info.t_done_ms
+        st = srv.stats()
+        assert st["slices"] > 1 and st["photons"] > 0
+    finally:
+        srv.close()
+
+
+def test_serving_failure_is_retried_deterministically(ctx, small_meshes):
+    _, mid, p, s, seed = small_meshes["sphere"]
+    ref = ctx.run_bscan(mid, p, s, seed)[0]
+    srv = O.Server(ctx, fail_request_index=0, slice_photons=2048)
+    try:
+        info, out = srv.wait(srv.submit(mid, p, s, seed), 120)
+        assert info.status == 0 and np.array_equal(out, ref)
+    finally:
+        srv.close()
+
+
+def test_serving_cancel_and_admission(ctx, small_meshes):
+    _, mid, p, s, seed = small_meshes["slab"]
+    big = [O.ascan(x.x, x.y, x.z, x.root_tet, 4_000_000, x.stream_salt) for x in s]
+    words = O.tally_words(p.num_depth_bins)
+    # budget fits one request's tallies at a time: the second waits for admission
+    srv = O.Server(ctx, hbm_budget_bytes=len(s) * words * 8, slice_photons=8192)
+    try:
+        with pytest.raises(O.OctmpsError) as e:
+            srv.submit(mid, p, s + s, seed)  # twice the budget: rejected at submit
+        assert e.value.status == 7
+        r1 = srv.submit(mid, p, big, seed)
+        r2 = srv.submit(mid, p, s, seed)
+        time.sleep(0.05)
+        i2 = srv.poll(r2)
+        assert not i2.done
+        srv.cancel(r1)
+        info1, _ = srv.wait(r1, 120)
+        assert info1.status == 10
+        info2, out2 = srv.wait(r2, 120)
+        assert info2.status == 0
+        assert np.array_equal(out2, ctx.run_bscan(mid, p, s, seed)[0])
+        assert info2.t_admit_ms >= info1.t_done_ms - 1e-6
+    finally:
+        srv.close()
+
+
+def test_translation_invariance_in_layers(ctx):
+    # In a laterally uniform layered medium, A-scans away from the sides are statistically
+    # the same (independent streams per A-scan).
+    m, p, scans, seed = configs.slab4(cells=(32, 32, 16), n_scans=2, photons=60000, bins=128, pitch=0.08)
+    mid = ctx.upload(m)
+    t, st = ctx.run_bscan(mid, p, scans, seed)
+    assert st.killed == 0
+    a, b = (O.decode_tally(x, p.num_depth_bins) for x in t)
+    n = scans[0].n_photons
+    ma, sa = bin_stats(a["sum"], a["sumsq"], n)
+    mb, sb = bin_stats(b["sum"], b["sumsq"], n)
+    sig = np.sqrt(sa ** 2 + sb ** 2)
+    pop = sig > 0
+    z = (ma[pop] - mb[pop]) / sig[pop]
+    assert np.mean(np.abs(z) > 3) <= 0.02 and np.abs(z).max() < 6
+    ra, rb = ma.sum(), mb.sum()
+    sr = np.sqrt((sa ** 2).sum() + (sb ** 2).sum())
+    assert abs(ra - rb) < 5 * sr
+
+
+def test_autotune_picks_a_valid_configuration(ctx, small_meshes):
+    _, mid, p, s, seed = small_meshes["slab"]
+    o, rate = ctx.autotune(mid, p, s, seed, calib=4000)
+    assert o.block_size in (64, 128, 256) and o.refill_quantum in (1, 4, 8, 16, 32) and rate > 0
+    t1 = ctx.run_bscan(mid, p, s, seed)[0]
+    t2 = ctx.run_bscan(mid, p, s, seed, opts=o)[0]
+    assert np.array_equal(t1, t2)  # launch shape never changes results
+
+
+def test_bump_projected_probes_run(ctx):
+    m, p, scans, seed = configs.bump(cells=(32, 32, 16), n_scans=3, photons=2000, bins=256)
+    mid = ctx.upload(m)
+    assert scans[1].z < 0  # on the bump, above z = 0
+    t, st = ctx.run_bscan(mid, p, scans, seed)
+    assert st.killed == 0 and st.photons == 6000
+    assert all(O.decode_tally(x, 256)["num_filtered"] > 0 for x in t)
+
+
+def _write_case(d, mesh):
+    (d / "case.opt").write_text(
+        "1.0  # version\n1  # runs\nout.mco A\n3000  # photons\n2  # regions\n1.0  # ambient n\n"
+        "1.40 1.0 100 0.9\n1.33 0.4 60 0.9\n-0.002 0.002 0.003  # start step end\n")
+    (d / "case.mci").write_text("1\n37\n0.925\n0.0\n0.06\n0.0015\n0\n10e-4\n5\n0.5\n0.01\n")
+    mesh.write(str(d / "case.mesh"))
+
+
+def test_cli_writes_reference_outputs(tmp_path, ctx):
+    mesh = O.Mesh.generate(O.GEOM_LAYERS, (16, 16, 8), (0.2, 0.2, 0.1), p=(0.02,), n_layers=2)
+    _write_case(tmp_path, mesh)
+    r = subprocess.run([O.CLI_PATH, "-S985456376", "-G1", "case.opt", "case.mesh", "case.mci"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    p, _ = O.read_opt(str(tmp_path / "case.opt"))
+    O.read_mci(str(tmp_path / "case.mci"), p)
+    B = p.num_depth_bins
+    for f in ("ClassI_Scatt.out", "ClassII_Scatt.out", "ClassI_ScattFilt.out", "ClassII_ScattFilt.out"):
+        rows = (tmp_path / f).read_text().splitlines()
+        assert len(rows) == 3 * B  # x = -0.002, 0, 0.002 (x < end, src/main.cu:948-950)
+    # the CLI's first A-scan equals the library's plain run with salt 0
+    m2 = O.Mesh.read(str(tmp_path / "case.mesh"))
+    mid = ctx.upload(m2)
+    root, _ = m2.find_root(-0.002, 0.0, 0.0)
+    t, _ = ctx.run_ascan(mid, p, O.ascan(-0.002, 0.0, 0.0, root, 3000), 985456376)
+    out = O.finalize(t, B, 3000, p.max_relative_contribution)
+    rows = [r.split(",") for r in (tmp_path / "ClassI_Scatt.out").read_text().splitlines()[:B]]
+    got = np.array([float(r[2]) for r in rows])
+    assert np.allclose(got, out[0, 0, 0], rtol=1e-5, atol=0)
