@@ -235,7 +235,7 @@ typedef struct {
     int32_t blocks_per_cu;   /* 0 = occupancy maximum */
     int32_t use_graph;       /* capture the batched launch in a hipGraph */
     int32_t autotune;        /* pick block size / quantum on a calibration slice */
-    int32_t reserved;
+    int32_t scatter_quantum; /* lanes of a wave parked for scattering before it runs; 0 = default */
     uint64_t max_steps_per_photon; /* 0 = default guard (1<<22) */
 } octmps_batch_opts;
 
@@ -325,6 +325,10 @@ octmps_status octmps_probe_fresnel(octmps_ctx *ctx, int32_t n, const float *ni, 
  * (kind 1, coefficient a) for uniform draws u in (0,1]. */
 octmps_status octmps_probe_sampler(octmps_ctx *ctx, int32_t kind, float param, int32_t n,
                                    const float *u, float *cost_out);
+
+/* Device-to-device copy of `bytes` through a plain streaming kernel (no counterpart in the
+ * reference): the calibration load for hardware byte counters. Returns the kernel time. */
+octmps_status octmps_probe_copy(octmps_ctx *ctx, uint64_t bytes, int32_t repeats, double *ms);
 
 #ifdef __cplusplus
 }

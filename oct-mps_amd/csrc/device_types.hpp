@@ -128,6 +128,8 @@ struct KArgs {
     uint32_t slice_mode;        // 1: save live photons and exit when ids run out
     uint32_t restore;           // 1: start from the saved lane states
     uint32_t exit_on_drain;     // slice mode: leave as soon as the ids run out
+    uint32_t scatter_q;         // parked lanes per wave before the scattering code runs
+    uint32_t pad2;
 };
 
 // Launchers implemented in transport.hip.
@@ -143,5 +145,6 @@ hipError_t launch_probe_fresnel(int n, const float *ni, const float *nt, const f
                                 hipStream_t stream);
 hipError_t launch_probe_sampler(int kind, float param, int n, const float *u, float *out,
                                 hipStream_t stream);
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 
 }  // namespace octmps

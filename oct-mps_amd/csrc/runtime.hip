@@ -144,6 +144,7 @@ octmps_status make_scan(octmps_ctx *ctx, const MeshEntry &me, const octmps_param
 
 int default_block(octmps_ctx *ctx) { return ctx->tuned_block > 0 ? ctx->tuned_block : 256; }
 int default_quantum(octmps_ctx *ctx) { return ctx->tuned_q > 0 ? ctx->tuned_q : 8; }
+int default_scatter(octmps_ctx *ctx) { return ctx->tuned_sq > 0 ? ctx->tuned_sq : 32; }
 
 void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid) {
     int occ = transport_blocks_per_cu(block, multi, false);
@@ -239,6 +240,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     A.n_seg = (int32_t)seg_slot.size();
     A.n_triplets = ctx->table->n_triplets;
     A.refill_q = (uint32_t)q;
+    A.scatter_q = (uint32_t)((opts && opts->scatter_quantum > 0) ? opts->scatter_quantum : default_scatter(ctx));
     const bool use_graph = opts && opts->use_graph && !plog_out;
     // Graph mode records the events around the graph launch (the fill kernels add a few us);
     // eager mode records them around the transport kernel alone.
@@ -480,29 +482,47 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     uint64_t per = std::max<uint64_t>(1, calib / (uint64_t)n);
     for (auto &s : cs) s.n_photons = std::min<uint64_t>(s.n_photons, per);
     std::vector<uint64_t> t((size_t)n * tally_words(p->num_depth_bins));
-    const int blocks[] = {64, 128, 256};
-    const int quanta[] = {1, 4, 8, 16, 32};
+    // Coordinate search: scatter quantum, then block size, then refill quantum, each with the
+    // best values found so far for the others (12 calibration launches).
     double best = -1;
     octmps_batch_opts win = *opts;
-    for (int b : blocks)
-        for (int q : quanta) {
-            octmps_batch_opts o = *opts;
-            o.block_size = b;
-            o.refill_quantum = q;
-            o.use_graph = 0;
-            octmps_run_stats st;
-            octmps_status s = run_scans(c, mesh_id, p, n, cs.data(), seed, &o, t.data(), &st, nullptr, 0);
-            if (s != OCTMPS_OK) return s;
-            double r = st.kernel_ms > 0 ? st.photons / (st.kernel_ms * 1e-3) : 0;
-            if (r > best) {
-                best = r;
-                win = o;
-            }
+    win.block_size = 256;
+    win.refill_quantum = 8;
+    win.scatter_quantum = 32;
+    win.use_graph = 0;
+    auto trial = [&](octmps_batch_opts o) -> octmps_status {
+        octmps_run_stats st;
+        octmps_status s = run_scans(c, mesh_id, p, n, cs.data(), seed, &o, t.data(), &st, nullptr, 0);
+        if (s != OCTMPS_OK) return s;
+        double r = st.kernel_ms > 0 ? st.photons / (st.kernel_ms * 1e-3) : 0;
+        if (r > best) {
+            best = r;
+            win = o;
         }
+        return OCTMPS_OK;
+    };
+    octmps_status s;
+    for (int sq : {1, 16, 32, 48}) {
+        octmps_batch_opts o = win;
+        o.scatter_quantum = sq;
+        if ((s = trial(o)) != OCTMPS_OK) return s;
+    }
+    for (int b : {64, 128, 256}) {
+        octmps_batch_opts o = win;
+        o.block_size = b;
+        if ((s = trial(o)) != OCTMPS_OK) return s;
+    }
+    for (int q : {1, 4, 8, 16, 32}) {
+        octmps_batch_opts o = win;
+        o.refill_quantum = q;
+        if ((s = trial(o)) != OCTMPS_OK) return s;
+    }
     opts->block_size = win.block_size;
     opts->refill_quantum = win.refill_quantum;
+    opts->scatter_quantum = win.scatter_quantum;
     c->tuned_block = win.block_size;
     c->tuned_q = win.refill_quantum;
+    c->tuned_sq = win.scatter_quantum;
     if (rate) *rate = best;
     return OCTMPS_OK;
 }
@@ -555,4 +575,33 @@ extern "C" octmps_status octmps_probe_sampler(octmps_ctx *c, int32_t kind, float
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(d);
     return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_sampler");
+}
+
+extern "C" octmps_status octmps_probe_copy(octmps_ctx *c, uint64_t bytes, int32_t repeats, double *ms) {
+    if (!c || bytes < 16 || repeats <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
+    std::lock_guard<std::mutex> lk(c->run_mu);
+    hipSetDevice(c->device);
+    void *a = nullptr, *b = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&a, bytes)) != hipSuccess) return hip_fail(e, "probe allocation");
+    if ((e = hipMalloc(&b, bytes)) != hipSuccess) {
+        (void)hipFree(a);
+        return hip_fail(e, "probe allocation");
+    }
+    e = hipMemsetAsync(a, 1, bytes, c->stream);
+    float total = 0;
+    for (int i = 0; i < repeats && e == hipSuccess; ++i) {
+        if ((e = hipEventRecord(c->ev0, c->stream)) != hipSuccess) break;
+        if ((e = launch_copy(a, b, bytes, c->stream)) != hipSuccess) break;
+        if ((e = hipEventRecord(c->ev1, c->stream)) != hipSuccess) break;
+        if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) break;
+        float t = 0;
+        (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
+        total += t;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (ms) *ms = total / repeats;
+    return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_copy");
 }

@@ -72,7 +72,7 @@ struct octmps_ctx {
     std::mutex run_mu;
     octmps::WorkBuffers work;
     octmps::GraphCache graph;
-    int tuned_block = 0, tuned_q = 0;
+    int tuned_block = 0, tuned_q = 0, tuned_sq = 0;
 };
 
 namespace octmps {

@@ -294,6 +294,7 @@ void scheduler(octmps_server *s) {
         A.n_seg = (int32_t)seg_slot.size();
         A.n_triplets = ctx->table->n_triplets;
         A.refill_q = (uint32_t)s->q;
+        A.scatter_q = 32;
         A.slice_mode = 1;
         A.restore = s->have_saved ? 1u : 0u;
         A.exit_on_drain = total > 0 ? 1u : 0u;

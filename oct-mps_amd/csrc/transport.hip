@@ -28,7 +28,9 @@ constexpr float kWeightF = 1e-4f;       // WEIGHT, src/octmps.h:22
 constexpr float kChanceF = 0.1f;        // CHANCE, src/octmps.h:23
 constexpr float kNormalSin2 = 1e-12f;
 
-constexpr uint32_t F_ALIVE = 1u, F_FST = 2u, F_NBS = 4u, F_CNBS = 8u;
+// Lane flags: photon alive, first backward bias done, a backward specular reflection seen,
+// the continuation photon's copy of that bit, scatter pending.
+constexpr uint32_t F_ALIVE = 1u, F_FST = 2u, F_NBS = 4u, F_CNBS = 8u, F_PEND = 16u;
 
 // __uint2float_rz(u) / 2^32: keep the top 24 significant bits, then an exact scale.
 __device__ __forceinline__ float unit_rz24(uint32_t u) {
@@ -45,9 +47,13 @@ __device__ __forceinline__ float rand_co(uint64_t &x, uint32_t a) {
 }
 __device__ __forceinline__ float rand_oc(uint64_t &x, uint32_t a) { return 1.0f - rand_co(x, a); }
 
+// Hardware reciprocal (v_rcp_f32, 1 ulp); the reference divides with round-down FP64
+// division (__ddiv_rd), so no FP32 rounding mode is "the" reference one.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Fresnel reflectance of FastReflectTransmit, src/octmps_kernel.cuh:608-645.
 __device__ __forceinline__ float fresnel_f(float ni, float nt, float ca1) {
-    float ni_nt = ni / nt;
+    float ni_nt = ni * rcp(nt);
     bool normal = ca1 > kCosZeroF;
     float sa1 = normal ? 0.0f : sqrtf(fmaxf(1.0f - ca1 * ca1, 0.0f));
     float sa2 = fminf(ni_nt * sa1, 1.0f);
@@ -56,7 +62,7 @@ __device__ __forceinline__ float fresnel_f(float ni, float nt, float ca1) {
     float sa1ca2 = normal ? 1.0f : sa1 * ca2;
     float ca1sa2 = normal ? ni_nt : ca1 * sa2;
     float cam = ca1ca2 + sa1sa2, sap = sa1ca2 + ca1sa2, sam = sa1ca2 - ca1sa2;
-    float r = sam / (sap * cam);
+    float r = sam * rcp(sap * cam);
     r *= r;
     r *= (ca1ca2 * ca1ca2 + sa1sa2 * sa1sa2);
     if (ca1 < kCos90F || sa2 == 1.0f) r = 1.0f;
@@ -66,7 +72,7 @@ __device__ __forceinline__ float fresnel_f(float ni, float nt, float ca1) {
 // SpinTheta, src/octmps_kernel.cuh:807-820 (hg_a = 1-g^2, hg_b = 1/(2g)).
 __device__ __forceinline__ float hg_cos(float g, float hg_a, float hg_b, float u_oc) {
     if (g == 0.0f) return 2.0f * u_oc - 1.0f;
-    float t = hg_a / (1.0f - g + 2.0f * g * u_oc);
+    float t = hg_a * rcp(1.0f - g + 2.0f * g * u_oc);
     float c = (1.0f + g * g - t * t) * hg_b;
     return fminf(fmaxf(c, -1.0f), 1.0f);
 }
@@ -76,7 +82,7 @@ __device__ __forceinline__ float bias_cos(float a, float inv_1ma, float inv_sq, 
                                           float opa2, float u_oc) {
     if (a == 0.0f) return u_oc;
     float t = u_oc * inv_1ma + (1.0f - u_oc) * inv_sq;
-    float c = (opa2 - 1.0f / (t * t)) * inv_2a;
+    float c = (opa2 - rcp(t * t)) * inv_2a;
     return fminf(fmaxf(c, -1.0f), 1.0f);
 }
 
@@ -87,8 +93,8 @@ __device__ __forceinline__ float bias_cos(float a, float inv_1ma, float inv_sq, 
 __device__ __forceinline__ void rotate(float ax, float ay, float az, float cost, float r_psi,
                                        int copysign_mode, float &ux, float &uy, float &uz) {
     float sint = sqrtf(fmaxf(1.0f - cost * cost, 0.0f));
-    float sinp, cosp;
-    sincospif(2.0f * r_psi, &sinp, &cosp);
+    // psi = 2 pi r: v_sin_f32 / v_cos_f32 take their argument in revolutions.
+    const float sinp = __builtin_amdgcn_sinf(r_psi), cosp = __builtin_amdgcn_cosf(r_psi);
     float stcp = sint * cosp, stsp = sint * sinp;
     float sp2 = ax * ax + ay * ay;
     float nx, ny, nz;
@@ -288,12 +294,47 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
         }
         if (A.slice_mode && ((drained && A.exit_on_drain) || (A.iter_budget && iters >= A.iter_budget))) break;
         ++iters;
-        wave_steps += (uint64_t)__popcll(amask);
-        if (!(flags & F_ALIVE)) continue;
+        // A lane whose step ended inside the tet (absorb + scatter) parks with F_PEND until
+        // enough lanes of the wave need the scattering code (scatter quantum), while the
+        // other lanes keep crossing faces. Each photon still runs its operations in the
+        // reference's order, so results do not depend on the quantum.
+        const bool ready = (flags & (F_ALIVE | F_PEND)) == F_ALIVE;
+        wave_steps += (uint64_t)__popcll(__ballot(ready));
 
         const DevSet &S = A.sets[set];
         bool finish = false, killed = false;
-        if (++steps > A.max_steps) {
+
+        // Roulette, src/octmps_kernel.cuh:1376-1392; returns true when the photon is done.
+        auto roulette = [&]() -> bool {
+            if (!(w < kWeightF)) return false;
+            const float r = rand_co(sG, aG);
+            if (w != 0.0f && r < kChanceF) {
+                w *= 10.0f;
+                return false;
+            }
+            if ((flags & F_FST) && S.type_bias != 3) {
+                const float lrt = lr_fst;
+                x = cx; y = cy; z = cz; ux = cux; uy = cuy; uz = cuz; w = cw;
+                op = cop; maxz = cmaxz; tet = ctet;
+                tau = 0.0f;
+                flags = F_ALIVE | ((flags & F_CNBS) ? F_NBS : 0u);
+                loc_fst = -1.0f;
+                if (cregion != region) {
+                    region = cregion;
+                    load_region(region);
+                }
+                const DevRegion &RC = A.regions[S.region_base + region];
+                const float cost = hg_cos(RC.g, RC.hg_a, RC.hg_b, rand_oc(sS, aS));
+                rotate(ux, uy, uz, cost, rand_co(sS, aS), 0, ux, uy, uz);
+                lr = (lrt < 1.0f) ? 1.0f - lrt : 1.0f;
+                return false;
+            }
+            return true;
+        };
+
+        if (!ready) {
+            // parked or dead lane: nothing to do in the crossing phase
+        } else if (++steps > A.max_steps) {
             finish = true;
             killed = true;
         } else {
@@ -316,7 +357,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
 #define OCTMPS_FACE(I, PL, C)                                            \
     if (C < 0.0f) {                                                     \
         const float perp = PL.x * x + PL.y * y + PL.z * z + PL.w;       \
-        const float d = -perp / C;                                      \
+        const float d = -perp * rcp(C);                                 \
         if (d < dmin) {                                                 \
             dmin = d;                                                   \
             imin = I;                                                   \
@@ -397,7 +438,18 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                     uy = ry;
                     uz = rz;
                 }
+                finish = roulette();
             } else {
+                flags |= F_PEND;
+            }
+        }
+
+        // ---- Scatter phase: Drop + Spin/SpinBias + roulette for the parked lanes.
+        const uint64_t pmask = __ballot((flags & F_PEND) != 0);
+        if (pmask != 0 &&
+            ((uint32_t)__popcll(pmask) >= A.scatter_q || __ballot((flags & (F_ALIVE | F_PEND)) == F_ALIVE) == 0)) {
+            if (flags & F_PEND) {
+                flags &= ~F_PEND;
                 // ---- Drop, src/octmps_kernel.cuh:584-588
                 const DevRegion RG = A.regions[S.region_base + region];
                 w -= w * RG.mua_muas;
@@ -423,15 +475,15 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                         first = true;
                     }
                     float vx = px - x, vy = py - y, vz = pz - z;
-                    const float len = sqrtf(vx * vx + vy * vy + vz * vz);
-                    vx = vx / len;
-                    vy = vy / len;
-                    vz = vz / len;
+                    const float len2 = vx * vx + vy * vy + vz * vz;
+                    const float rl = rsqrtf(len2);
+                    vx *= rl;
+                    vy *= rl;
+                    vz *= rl;
                     if (((flags & F_FST) || (flags & F_NBS)) && F == 0) {
                         F = 2;
-                        const float next = S.next_step_num / muas;
-                        const float dxp = x - px, dyp = y - py, dzp = z - pz;
-                        const float dist = sqrtf(dxp * dxp + dyp * dyp + dzp * dzp);
+                        const float next = S.next_step_num * rcp(muas);
+                        const float dist = len2 * rl;  // |x - probe|
                         if (next >= dist && -vz >= S.cos_angle) F = 1;
                         atmp = S.bias_a;
                     }
@@ -460,46 +512,23 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                             // Equation (8), src/octmps_kernel.cuh:1047-1050.
                             const float t1 = S.opa2 - 2.0f * atmp * cost;
                             const float t2 = 1.0f + g * g - 2.0f * g * costg;
-                            factor = RG.lr8 * (t1 * sqrtf(t1)) / (t2 * sqrtf(t2));
+                            factor = RG.lr8 * (t1 * sqrtf(t1)) * rcp(t2 * sqrtf(t2));
                         } else {
                             // Equation (9), src/octmps_kernel.cuh:1071-1078.
                             const float costg1 = vx * ux + vy * uy + vz * uz;
                             const float k1 = sel == 1 ? cost : costg1;
                             const float k2 = sel == 1 ? costg : cost;
                             const float t1 = S.opa2 - 2.0f * atmp * k1;
-                            const float pdf1 = S.pdfb_k / (t1 * sqrtf(t1));
+                            const float pdf1 = S.pdfb_k * rcp(t1 * sqrtf(t1));
                             const float t2 = sqrtf(1.0f + g * g - 2.0f * g * k2);
-                            const float pdf2 = RG.hg_a / (2.0f * (t2 * t2 * t2));
-                            factor = pdf2 / (S.p_add * pdf1 + (1.0f - S.p_add) * pdf2);
+                            const float pdf2 = RG.hg_a * rcp(2.0f * (t2 * t2 * t2));
+                            factor = pdf2 * rcp(S.p_add * pdf1 + (1.0f - S.p_add) * pdf2);
                         }
                         lr *= factor;
                         if (first) lr_fst = lr;
                     }
                 }
-            }
-            // ---- Roulette, src/octmps_kernel.cuh:1376-1392
-            if (w < kWeightF) {
-                const float r = rand_co(sG, aG);
-                if (w != 0.0f && r < kChanceF) {
-                    w *= 10.0f;
-                } else if ((flags & F_FST) && S.type_bias != 3) {
-                    const float lrt = lr_fst;
-                    x = cx; y = cy; z = cz; ux = cux; uy = cuy; uz = cuz; w = cw;
-                    op = cop; maxz = cmaxz; tet = ctet;
-                    tau = 0.0f;
-                    flags = F_ALIVE | ((flags & F_CNBS) ? F_NBS : 0u);
-                    loc_fst = -1.0f;
-                    if (cregion != region) {
-                        region = cregion;
-                        load_region(region);
-                    }
-                    const DevRegion &RC = A.regions[S.region_base + region];
-                    const float cost = hg_cos(RC.g, RC.hg_a, RC.hg_b, rand_oc(sS, aS));
-                    rotate(ux, uy, uz, cost, rand_co(sS, aS), 0, ux, uy, uz);
-                    lr = (lrt < 1.0f) ? 1.0f - lrt : 1.0f;
-                } else {
-                    finish = true;
-                }
+                finish = roulette();
             }
         }
         if (finish) {
@@ -637,4 +666,19 @@ hipError_t launch_probe_sampler(int kind, float p, int n, const float *u, float 
     return hipGetLastError();
 }
 
+}  // namespace octmps
+
+namespace octmps {
+namespace {
+__global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream) {
+    uint64_t n = bytes / sizeof(uint4);
+    hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, stream, (const uint4 *)src, (uint4 *)dst, n);
+    return hipGetLastError();
+}
 }  // namespace octmps

@@ -96,7 +96,7 @@ class RunStats(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [
         ("block_size", C.c_int32), ("refill_quantum", C.c_int32), ("blocks_per_cu", C.c_int32),
-        ("use_graph", C.c_int32), ("autotune", C.c_int32), ("reserved", C.c_int32),
+        ("use_graph", C.c_int32), ("autotune", C.c_int32), ("scatter_quantum", C.c_int32),
         ("max_steps_per_photon", C.c_uint64),
     ]
 
@@ -180,6 +180,7 @@ SIGNATURES = [
                                        C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("octmps_probe_sampler", C.c_int, [_P, C.c_int32, C.c_float, C.c_int32, C.POINTER(C.c_float),
                                        C.POINTER(C.c_float)]),
+    ("octmps_probe_copy", C.c_int, [_P, C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
 ]
 
 
@@ -434,6 +435,11 @@ class Context:
         r = np.empty(len(ni), np.float32)
         check(lib.octmps_probe_fresnel(self.h, len(ni), _fp(ni), _fp(nt), _fp(ca1), _fp(r)), "probe_fresnel")
         return r
+
+    def probe_copy(self, nbytes, repeats=3):
+        ms = C.c_double(0)
+        check(lib.octmps_probe_copy(self.h, nbytes, repeats, C.byref(ms)), "probe_copy")
+        return ms.value
 
     def probe_sampler(self, kind, param, u):
         u = np.ascontiguousarray(u, np.float32)

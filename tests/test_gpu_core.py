@@ -119,3 +119,17 @@ def test_graph_replay_matches_eager(ctx):
         assert np.array_equal(te, tg)
         ta, _ = ctx.run_ascan(mid, params, scans[0], seed)  # eager work between replays
         assert np.array_equal(ta, te[0])
+
+
+@pytest.mark.parametrize("sq,q,block", [(1, 1, 64), (16, 8, 128), (64, 32, 256), (48, 4, 192)])
+def test_launch_shape_never_changes_results(ctx, sq, q, block):
+    # Scatter quantum, refill quantum and block size only reorder work between lanes;
+    # every photon runs the same operations on the same streams.
+    mesh, params, scans, seed = configs.sphere(cells=12, photons=3000, bins=96)
+    mid = ctx.upload(mesh)
+    ref, _ = ctx.run_bscan(mid, params, scans, seed)
+    o = O.BatchOpts()
+    o.scatter_quantum, o.refill_quantum, o.block_size = sq, q, block
+    got, st = ctx.run_bscan(mid, params, scans, seed, opts=o)
+    assert st.block_size == block and st.refill_quantum == q
+    assert np.array_equal(ref, got)
