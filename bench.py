@@ -31,7 +31,65 @@ def parse():
     ap.add_argument("--cpu-photons", type=int, default=3000, help="oracle photons per sampled A-scan")
     ap.add_argument("--cpu-scans", type=int, default=8)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-window", type=float, default=0.2, help="seconds of arrivals in the served stream")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--slab128-cells", type=int, default=128)
     return ap.parse_args()
+
+
+def run_serving(O, configs, ctx, a, log):
+    """Served stream: Poisson arrivals (seed 11) over four resident meshes; each request is
+    1-256 A-scans with log-uniform budgets in [1e4, 1e6]. Latency = submit -> result."""
+    import threading
+
+    import numpy as np
+
+    meshes = []
+    t0 = time.time()
+    m, p, s, _ = configs.sphere(photons=1)
+    meshes.append((m, ctx.upload(m), p, s[0]))
+    m, p, s, _ = configs.slab4(n_scans=1, photons=1)
+    meshes.append((m, ctx.upload(m), p, s[0]))
+    m, p, s, _ = configs.ellipsoid(n_scans=1, budgets=[1])
+    meshes.append((m, ctx.upload(m), p, s[0]))
+    c = a.slab128_cells
+    m, p = configs.slab128(cells=(c, c, c))
+    root, z = m.find_root(0.0, 0.0, 0.0)
+    meshes.append((m, ctx.upload(m), p, O.ascan(0.0, 0.0, z, root, 1)))
+    log(f"served meshes resident ({sum(x[0].num_tets for x in meshes)} tets) in {time.time() - t0:.1f}s")
+    trace = configs.served_stream(a.serve_rate, a.serve_window, seed=11, n_meshes=len(meshes))
+    srv = O.Server(ctx)
+    ids = []
+    photons = 0
+    start = time.perf_counter()
+    for i, (t_arr, mi, n, budgets) in enumerate(trace):
+        mesh, mid, p, probe = meshes[mi]
+        scans = [O.ascan(probe.x, probe.y, probe.z, probe.root_tet, int(b), salt=1000 * i + k)
+                 for k, b in enumerate(budgets)]
+        photons += int(sum(budgets))
+        delay = start + t_arr - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(mid, p, scans, 1))
+    lat = []
+    first = last = None
+    for rid in ids:
+        info, _ = srv.wait(rid, -1.0)
+        assert info.status == 0
+        lat.append(info.t_done_ms - info.t_submit_ms)
+        first = info.t_submit_ms if first is None else min(first, info.t_submit_ms)
+        last = info.t_done_ms if last is None else max(last, info.t_done_ms)
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    span_s = (last - first) * 1e-3
+    return {"requests": len(ids), "rate_req_s": a.serve_rate, "arrival_window_s": a.serve_window,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "photons": photons, "photons_per_s": photons / span_s, "span_s": span_s,
+            "slices": st["slices"], "kernel_busy_frac": st["kernel_ms"] * 1e-3 / span_s,
+            "meshes": "sphere 196,608 + slab4 786,432 + ellipsoid 786,432 + "
+                      f"slab128 {meshes[3][0].num_tets} tets, all resident"}
 
 
 def main():
@@ -114,6 +172,12 @@ def main():
                          f"(FP64 oracle, {threads} threads, {cw:.1f} s)"}
         print(f"[bench] cpu oracle: {n} photons in {cw:.1f}s", file=sys.stderr, flush=True)
 
+    serving = None
+    if not a.no_serve:
+        serving = run_serving(O, configs, ctx, a, lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True))
+        print(f"[bench] served stream: {serving['requests']} requests, p50 {serving['p50_ms']:.1f} ms, "
+              f"p99 {serving['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+
     ctx.close()
     line = {
         "metric": "photons/s per B-scan, batched vs plain kernel; p50/p99 A-scan request latency",
@@ -140,6 +204,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step"},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 
