@@ -162,7 +162,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     if (!ctx || !p || (n > 0 && !scans) || !tallies_out) return fail(OCTMPS_E_INVALID, "null argument");
     if (n <= 0) return fail(OCTMPS_E_INVALID, "no A-scans");
     std::lock_guard<std::mutex> lk(ctx->run_mu);
-    std::lock_guard<std::mutex> pk(ctx->pool_mu);
+    std::shared_lock<std::shared_mutex> pk(ctx->pool_mu);
     if (mesh_id < 0 || mesh_id >= (int)ctx->meshes.size() || !ctx->meshes[mesh_id].live)
         return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
     hipSetDevice(ctx->device);
@@ -360,7 +360,7 @@ extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh
     const HostMesh &m = mh->m;
     const int64_t T = m.ntets();
     if (m.max_region >= 255) return fail(OCTMPS_E_RANGE, "regions must be < 255");
-    std::lock_guard<std::mutex> pk(c->pool_mu);
+    std::unique_lock<std::shared_mutex> pk(c->pool_mu);
     hipSetDevice(c->device);
     if (c->pool_used + T > (int64_t)INT32_MAX) return fail(OCTMPS_E_RANGE, "mesh pool exceeds 2^31 tets");
     hipError_t e;
@@ -427,7 +427,7 @@ extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh
 
 extern "C" octmps_status octmps_ctx_release_mesh(octmps_ctx *c, int32_t id) {
     if (!c) return fail(OCTMPS_E_INVALID, "null argument");
-    std::lock_guard<std::mutex> pk(c->pool_mu);
+    std::unique_lock<std::shared_mutex> pk(c->pool_mu);
     if (id < 0 || id >= (int)c->meshes.size() || !c->meshes[id].live) return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
     c->meshes[id].live = false;
     c->meshes[id].region.clear();

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -63,7 +64,7 @@ struct octmps_ctx {
     const octmps::StreamTable *table = nullptr;
     uint32_t *d_mult = nullptr;
     // Resident mesh pool (global tet ids).
-    std::mutex pool_mu;
+    std::shared_mutex pool_mu;  // shared: launches and submits; exclusive: mesh upload/release
     float4 *pool_planes = nullptr;
     octmps::TopoRec *pool_topo = nullptr;
     int64_t pool_cap = 0, pool_used = 0;

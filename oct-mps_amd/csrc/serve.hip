@@ -214,7 +214,7 @@ void scheduler(octmps_server *s) {
             if (s->stop && s->pending.empty() && s->active.empty()) break;
             admit(s, staged);
         }
-        std::lock_guard<std::mutex> pk(ctx->pool_mu);
+        std::shared_lock<std::shared_mutex> pk(ctx->pool_mu);
         for (Req *r : staged) {
             octmps_status st = stage(s, r);
             if (st != OCTMPS_OK) {
@@ -418,7 +418,7 @@ extern "C" octmps_status octmps_serve_submit(octmps_server *s, const octmps_requ
         return fail(OCTMPS_E_INVALID, "bad request");
     octmps_ctx *ctx = s->ctx;
     {
-        std::lock_guard<std::mutex> pk(ctx->pool_mu);
+        std::shared_lock<std::shared_mutex> pk(ctx->pool_mu);
         if (rq->mesh_id < 0 || rq->mesh_id >= (int)ctx->meshes.size() || !ctx->meshes[rq->mesh_id].live)
             return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
         DevSet S;
