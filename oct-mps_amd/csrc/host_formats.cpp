@@ -233,8 +233,9 @@ extern "C" uint64_t octmps_tally_words(int32_t b) { return b > 0 ? tally_words(b
 
 namespace {
 
-long double q64_to_ld(uint64_t lo, uint64_t hi) {
-    return (long double)hi + (long double)lo * 0x1p-64L;
+// Tally word pair: hi in units of 2^-32, lo in units of 2^-64 (see octmps.h).
+long double split_to_ld(uint64_t lo, uint64_t hi) {
+    return (long double)hi * 0x1p-32L + (long double)lo * 0x1p-64L;
 }
 
 struct Bin {
@@ -245,8 +246,8 @@ struct Bin {
 Bin get_bin(const uint64_t *t, int32_t B, int c, int b) {
     const uint64_t *base = t + 4 + (uint64_t)c * 6 * B;
     Bin r;
-    r.sum = (double)q64_to_ld(base[b], base[B + b]);
-    r.sumsq = (double)q64_to_ld(base[2 * B + b], base[3 * B + b]);
+    r.sum = (double)split_to_ld(base[b], base[B + b]);
+    r.sumsq = (double)split_to_ld(base[2 * B + b], base[3 * B + b]);
     r.count = base[4 * B + b];
     std::memcpy(&r.max, &base[5 * B + b], sizeof(double));
     return r;

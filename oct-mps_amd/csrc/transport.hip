@@ -6,8 +6,9 @@
 // that many photon ids with one atomic and relaunches those lanes (ballot + mbcnt ranks).
 // Photon ids map to A-scans through a sorted segment table, and every photon derives its
 // three MWC streams from (seed, salt, photon index), so results do not depend on lane,
-// launch or slice. Transport is FP32; tallies are exact Q64.64 integers (two 64-bit words
-// with a carry, like the reference's EMULATED_ATOMIC, src/octmps_kernel.cuh:42-46).
+// launch or slice. Transport is FP32; tallies are exact fixed-point integers (two 64-bit words
+// per value with a carry-free split, non-returning atomics; the reference's EMULATED_ATOMIC,
+// src/octmps_kernel.cuh:42-46, needed a returning add to propagate its carry).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -114,33 +115,34 @@ __device__ __forceinline__ void rotate(float ax, float ay, float az, float cost,
     uz = nz * rn;
 }
 
-// Exact Q64.64 fixed point of a non-negative float (truncated below 2^-64).
-__device__ __forceinline__ void to_q64(float c, uint64_t &lo, uint64_t &hi) {
+// Exact split fixed point of a non-negative float: v = floor(c * 2^64) = hi * 2^32 + lo with
+// lo < 2^32. The tally keeps two u64 accumulators, sum(hi) in units of 2^-32 and sum(lo) in
+// units of 2^-64; since each lo is < 2^32 neither word needs a carry, so both adds are
+// non-returning atomics and the total is exact for up to 2^32 adds per bin.
+__device__ __forceinline__ void to_split(float c, uint64_t &lo, uint64_t &hi) {
     lo = 0;
     hi = 0;
-    if (!(c > 0.0f) || !(c < 1.8e19f)) return;
+    if (!(c > 0.0f) || !(c < 4.0e9f)) return;
     uint32_t b = __float_as_uint(c);
     int e = (int)((b >> 23) & 0xffu);
     uint64_t m = b & 0x7fffffu;
     if (e == 0) e = 1;
     else m |= 0x800000u;
-    int sh = e - 86;  // c * 2^64 = m * 2^(e - 150 + 64)
+    int sh = e - 86;  // c * 2^64 = m * 2^(e - 150 + 64), m < 2^24
     if (sh <= -24) return;
     if (sh < 0) {
         lo = m >> (-sh);
-    } else if (sh < 64) {
-        lo = m << sh;
-        hi = (sh > 40) ? (m >> (64 - sh)) : 0;
+    } else if (sh < 32) {
+        uint64_t v = m << sh;  // < 2^56
+        lo = v & 0xffffffffull;
+        hi = v >> 32;
     } else {
-        hi = m << (sh - 64);
+        hi = m << (sh - 32);
     }
 }
 
-__device__ __forceinline__ void add_q64(uint64_t *lo_p, uint64_t *hi_p, uint64_t lo, uint64_t hi) {
-    if (lo) {
-        uint64_t old = atomicAdd((unsigned long long *)lo_p, (unsigned long long)lo);
-        if (old + lo < old) hi += 1;
-    }
+__device__ __forceinline__ void add_split(uint64_t *lo_p, uint64_t *hi_p, uint64_t lo, uint64_t hi) {
+    if (lo) atomicAdd((unsigned long long *)lo_p, (unsigned long long)lo);
     if (hi) atomicAdd((unsigned long long *)hi_p, (unsigned long long)hi);
 }
 
@@ -157,15 +159,15 @@ __device__ void tally_detection(const KArgs &A, const DevSet &S, uint32_t slot, 
     int cls = c1 ? 0 : 1;
     int B = S.num_bins;
     uint64_t lo, hi, slo, shi;
-    to_q64(c, lo, hi);
-    to_q64(c * c, slo, shi);
+    to_split(c, lo, hi);
+    to_split(c * c, slo, shi);
     uint64_t maxbits = (uint64_t)__double_as_longlong((double)c);
     uint64_t *base = T + 4 + (uint64_t)cls * 6 * (uint64_t)B;
     for (int ii = vp - 3; ii < vp + 3; ++ii) {
         if (ii >= 0 && ii < B) {
             if (ii == vp) atomicAdd((unsigned long long *)&T[1 + cls], 1ull);
-            add_q64(&base[ii], &base[B + ii], lo, hi);
-            add_q64(&base[2 * B + ii], &base[3 * B + ii], slo, shi);
+            add_split(&base[ii], &base[B + ii], lo, hi);
+            add_split(&base[2 * B + ii], &base[3 * B + ii], slo, shi);
             atomicAdd((unsigned long long *)&base[4 * B + ii], 1ull);
             atomicMax((unsigned long long *)&base[5 * B + ii], (unsigned long long)maxbits);
         }

@@ -352,8 +352,9 @@ def decode_tally(t, bins):
     for c in range(2):
         base = 4 + c * 6 * bins
         blk = t[base:base + 6 * bins].reshape(6, bins)
-        s.append(blk[1].astype(np.float64) + blk[0].astype(np.float64) * 2.0 ** -64)
-        sq.append(blk[3].astype(np.float64) + blk[2].astype(np.float64) * 2.0 ** -64)
+        # hi words in units of 2^-32, lo words in units of 2^-64 (include/octmps.h)
+        s.append(blk[1].astype(np.float64) * 2.0 ** -32 + blk[0].astype(np.float64) * 2.0 ** -64)
+        sq.append(blk[3].astype(np.float64) * 2.0 ** -32 + blk[2].astype(np.float64) * 2.0 ** -64)
         cnt.append(blk[4].astype(np.float64))
         mx.append(blk[5].view(np.float64).copy())
     out["sum"] = np.array(s)

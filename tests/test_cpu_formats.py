@@ -98,7 +98,8 @@ def test_mesh_serialisation_like_reference(case):
 
 
 def _tally_from(sums, sqs, maxs, cnts, nf):
-    """Integer tally whose Q64.64 sums are exactly the given doubles (multiples of 2^-30)."""
+    """Integer tally whose sums are exactly the given doubles (multiples of 2^-30): the high
+    word counts 2^-32 units, the low word 2^-64 units."""
     B = sums.shape[1]
     t = np.zeros(4 + 12 * B, np.uint64)
     t[0:3] = nf
@@ -106,9 +107,9 @@ def _tally_from(sums, sqs, maxs, cnts, nf):
         base = 4 + c * 6 * B
         for arr, off in ((sums[c], 0), (sqs[c], 2)):
             k = np.rint(arr * 2.0 ** 30).astype(np.uint64)
-            # value * 2^64 = k * 2^34: low word (k << 34) mod 2^64, high word k >> 30
-            t[base + off * B:base + (off + 1) * B] = (k & np.uint64(2 ** 30 - 1)) << np.uint64(34)
-            t[base + (off + 1) * B:base + (off + 2) * B] = k >> np.uint64(30)
+            # value = k * 2^-30 = (4k) * 2^-32: all in the high word, low word 0
+            t[base + off * B:base + (off + 1) * B] = 0
+            t[base + (off + 1) * B:base + (off + 2) * B] = k << np.uint64(2)
         t[base + 4 * B:base + 5 * B] = cnts[c]
         t[base + 5 * B:base + 6 * B] = maxs[c].view(np.uint64)
     return t

@@ -109,10 +109,11 @@ def test_tally_finalize_formula():
         base = 4 + c * 6 * B
         s = rng.integers(0, 2 ** 40, B).astype(np.uint64)
         q = (s.astype(np.float64) ** 2 / 2 ** 20 / n * rng.uniform(1, 3, B)).astype(np.uint64)
-        t[base + B:base + 2 * B] = s >> np.uint64(20)           # hi words
-        t[base:base + B] = (s & np.uint64(2 ** 20 - 1)) << np.uint64(44)
-        t[base + 3 * B:base + 4 * B] = q >> np.uint64(20)
-        t[base + 2 * B:base + 3 * B] = (q & np.uint64(2 ** 20 - 1)) << np.uint64(44)
+        # value s * 2^-20: high word (units 2^-32) = s * 2^12 - (low part), low word (2^-64 units)
+        t[base + B:base + 2 * B] = (s >> np.uint64(4)) << np.uint64(16)
+        t[base:base + B] = (s & np.uint64(15)) << np.uint64(44)
+        t[base + 3 * B:base + 4 * B] = (q >> np.uint64(4)) << np.uint64(16)
+        t[base + 2 * B:base + 3 * B] = (q & np.uint64(15)) << np.uint64(44)
         mx = (s.astype(np.float64) * 2.0 ** -20) * rng.uniform(0, 0.03, B)
         t[base + 5 * B:base + 6 * B] = mx.view(np.uint64)
         vals[c] = (s.astype(np.float64) * 2.0 ** -20, q.astype(np.float64) * 2.0 ** -20, mx)
