@@ -21,6 +21,11 @@ sys.path.insert(0, os.path.join(ROOT, "oct-mps_amd"))
 
 ALG_BYTES_PER_STEP = 84  # one tet record per photon-step: 4 planes x 16 B + 4 links x 4 B + region 4 B
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E nominal
+# Measured HBM bytes per photon-step of the batched slab4 launch: rocprofv3 --pmc FETCH_SIZE
+# (x2.0, the factor by which that counter under-reads a copy kernel of known size) plus
+# WRITE_SIZE (x1.0), divided by the launch's photon-steps. profiles/transport_slab4_pmc.md
+# holds the counter values and commands.
+TRAFFIC_B_PER_STEP = None
 
 
 def parse():
@@ -119,7 +124,7 @@ def main():
     copy = None
     if a.calibrate_copy_mb:
         nbytes = a.calibrate_copy_mb << 20
-        ms = ctx.probe_copy(nbytes, 3)
+        ms = ctx.probe_copy(nbytes, 10)
         copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB_s": 2 * nbytes / (ms * 1e-3) / 1e9}
         print(f"[bench] copy calibration: {nbytes} B in {ms:.3f} ms ({copy['GB_s']:.0f} GB/s r+w)",
               file=sys.stderr, flush=True)
@@ -137,13 +142,14 @@ def main():
         ctx.run_bscan(mid, params, scans, seed, opts=opts)
         print(f"[bench] warmup step {i} done", file=sys.stderr, flush=True)
     total_photons = sum(s.n_photons for s in scans)
-    walls, kms, steps = [], [], []
+    walls, kms, steps, iters = [], [], [], []
     for i in range(a.steps):
         ts = time.perf_counter()
         t, st = ctx.run_bscan(mid, params, scans, seed, opts=opts)
         walls.append(time.perf_counter() - ts)
         kms.append(st.kernel_ms)
         steps.append(st.photon_steps)
+        iters.append(st.wave_iterations)
         print(f"[bench] step {i}: {walls[-1]:.3f}s kernel {st.kernel_ms:.1f} ms, {st.photon_steps} photon-steps, "
               f"killed {st.killed}", file=sys.stderr, flush=True)
     wall = sum(walls)
@@ -218,12 +224,14 @@ def main():
                                f"512 bins to 1.5 mm, TypeBias 37",
                    "photons": total_photons, "path": "batched persistent kernel, hipGraph launch"},
         "batched": {"photons_per_s": value, "kernel_ms": kms, "photon_steps": steps,
-                    "steps_per_photon": steps_per_launch / total_photons},
+                    "steps_per_photon": steps_per_launch / total_photons,
+                    "lane_utilisation": sum(steps) / (64.0 * max(1, sum(iters)))},
         "plain": plain,
         "copy_calibration": copy,
         "batched_over_plain": (value / plain["photons_per_s"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (steps_per_launch * TRAFFIC_B_PER_STEP) if TRAFFIC_B_PER_STEP else None,
                      "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step"},
         "cpu_baseline": cpu,
         "serving": serving,

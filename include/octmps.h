@@ -229,6 +229,8 @@ typedef struct {
     uint32_t block_size;
     uint32_t grid_size;
     uint32_t refill_quantum;
+    uint64_t wave_iterations; /* loop iterations summed over waves; lane utilisation =
+                                 photon_steps / (64 * wave_iterations) */
 } octmps_run_stats;
 
 typedef struct {

@@ -79,20 +79,31 @@ struct DevScan {
 static_assert(sizeof(DevScan) == 64, "DevScan must be 64 bytes");
 
 // Live photon of one lane saved between slices (the reference's SaveThreadState,
-// src/octmps_kernel.cuh:1154-1224, as one record per lane).
+// src/octmps_kernel.cuh:1154-1224, as one record per lane). Its continuation photon stays
+// in the lane's ContRec, which persists across slices like this record.
 struct SavedLane {
     float x, y, z, ux, uy, uz, w, tau, op, maxz, lr, lr_fst, loc_fst;
     int32_t tet, region;
     uint32_t flags, steps;
     uint32_t aG, aR, aS;
     uint64_t sG, sR, sS;
-    float cx, cy, cz, cux, cuy, cuz, cw, cop, cmaxz;
-    int32_t ctet, cregion;
     uint32_t slot;
     uint64_t id;
     uint32_t ndet;
     float csum;
 };
+
+// Continuation photon (photon_cont, src/octmps_kernel.h:219-255) of one lane: the copy taken
+// at the first backward bias and restored by roulette. Written and read rarely, so it lives
+// in memory (one record per lane) rather than in eleven VGPRs.
+struct __align__(16) ContRec {
+    float x, y, z, w;
+    float ux, uy, uz, op;
+    float maxz;
+    int32_t tet, region;
+    uint32_t pad;
+};
+static_assert(sizeof(ContRec) == 48, "ContRec must be 48 bytes");
 
 struct PhotonLogRec {
     uint32_t steps;
@@ -102,7 +113,7 @@ struct PhotonLogRec {
 };
 
 // Control words.
-enum { CTRL_NEXT_ID = 0, CTRL_STEPS = 1, CTRL_KILLED = 2, CTRL_DONE = 3, CTRL_WORDS = 8 };
+enum { CTRL_NEXT_ID = 0, CTRL_STEPS = 1, CTRL_KILLED = 2, CTRL_DONE = 3, CTRL_ITERS = 4, CTRL_WORDS = 8 };
 
 struct KArgs {
     const float4 *planes;
@@ -119,6 +130,7 @@ struct KArgs {
     uint64_t *slot_done;        // photons completed per slot
     SavedLane *save;            // lane states between slices (slice mode)
     PhotonLogRec *plog;         // optional per-photon summaries, indexed by global id
+    ContRec *cont;              // continuation photon per lane (grid * block records)
     uint64_t id_end;            // ids [ctrl[NEXT], id_end) may be launched
     uint64_t iter_budget;       // slice mode: loop iterations per wave before saving
     uint64_t max_steps;         // per-photon step guard

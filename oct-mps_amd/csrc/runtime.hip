@@ -209,7 +209,8 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
         (e = W.seg_poff.ensure(seg_poff.size() * 8)) != hipSuccess ||
         (e = W.tallies.ensure((size_t)n * words * 8)) != hipSuccess ||
         (e = W.ctrl.ensure(CTRL_WORDS * 8)) != hipSuccess ||
-        (e = W.slot_done.ensure((size_t)n * 8)) != hipSuccess)
+        (e = W.slot_done.ensure((size_t)n * 8)) != hipSuccess ||
+        (e = W.cont.ensure((size_t)grid * block * sizeof(ContRec))) != hipSuccess)
         return hip_fail(e, "device allocation");
     if (plog_out && (e = W.plog.ensure(total * sizeof(PhotonLogRec))) != hipSuccess)
         return hip_fail(e, "photon log allocation");
@@ -235,6 +236,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     A.ctrl = (uint64_t *)W.ctrl.p;
     A.slot_done = (uint64_t *)W.slot_done.p;
     A.plog = plog_out ? (PhotonLogRec *)W.plog.p : nullptr;
+    A.cont = (ContRec *)W.cont.p;
     A.id_end = total;
     A.max_steps = (opts && opts->max_steps_per_photon) ? opts->max_steps_per_photon : (1ull << 22);
     A.n_seg = (int32_t)seg_slot.size();
@@ -300,6 +302,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
         stats->block_size = (uint32_t)block;
         stats->grid_size = (uint32_t)grid;
         stats->refill_quantum = (uint32_t)q;
+        stats->wave_iterations = ctrl[CTRL_ITERS];
     }
     return OCTMPS_OK;
 }
@@ -588,17 +591,16 @@ extern "C" octmps_status octmps_probe_copy(octmps_ctx *c, uint64_t bytes, int32_
         (void)hipFree(a);
         return hip_fail(e, "probe allocation");
     }
+    // One untimed copy, then `repeats` back-to-back copies between two events: a single
+    // short kernel between events would mostly time the launch.
     e = hipMemsetAsync(a, 1, bytes, c->stream);
+    if (e == hipSuccess) e = launch_copy(a, b, bytes, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    for (int i = 0; i < repeats && e == hipSuccess; ++i) e = launch_copy(a, b, bytes, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
     float total = 0;
-    for (int i = 0; i < repeats && e == hipSuccess; ++i) {
-        if ((e = hipEventRecord(c->ev0, c->stream)) != hipSuccess) break;
-        if ((e = launch_copy(a, b, bytes, c->stream)) != hipSuccess) break;
-        if ((e = hipEventRecord(c->ev1, c->stream)) != hipSuccess) break;
-        if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) break;
-        float t = 0;
-        (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
-        total += t;
-    }
+    if (e == hipSuccess) (void)hipEventElapsedTime(&total, c->ev0, c->ev1);
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(a);
     (void)hipFree(b);

@@ -22,14 +22,14 @@ struct DevBuf {
 };
 
 struct WorkBuffers {
-    DevBuf regions, sets, slots, seg_start, seg_slot, seg_poff, tallies, ctrl, slot_done, plog;
+    DevBuf regions, sets, slots, seg_start, seg_slot, seg_poff, tallies, ctrl, slot_done, plog, cont;
     void release_all() {
-        for (DevBuf *b : {&regions, &sets, &slots, &seg_start, &seg_slot, &seg_poff, &tallies, &ctrl, &slot_done, &plog})
+        for (DevBuf *b : {&regions, &sets, &slots, &seg_start, &seg_slot, &seg_poff, &tallies, &ctrl, &slot_done, &plog, &cont})
             b->release();
     }
     uint64_t bytes() const {
         uint64_t s = 0;
-        for (const DevBuf *b : {&regions, &sets, &slots, &seg_start, &seg_slot, &seg_poff, &tallies, &ctrl, &slot_done, &plog})
+        for (const DevBuf *b : {&regions, &sets, &slots, &seg_start, &seg_slot, &seg_poff, &tallies, &ctrl, &slot_done, &plog, &cont})
             s += b->cap;
         return s;
     }

@@ -79,7 +79,7 @@ struct octmps_server {
     // device state (owned by the scheduler thread)
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    DevBuf regions, sets, slots, seg_start, seg_slot, seg_poff, tallies, ctrl, slot_done, save;
+    DevBuf regions, sets, slots, seg_start, seg_slot, seg_poff, tallies, ctrl, slot_done, save, cont;
     int grid = 0, block = 0, q = 0;
     uint64_t lanes = 0;
     int n_slots = 0;
@@ -288,6 +288,7 @@ void scheduler(octmps_server *s) {
         A.ctrl = (uint64_t *)s->ctrl.p;
         A.slot_done = (uint64_t *)s->slot_done.p;
         A.save = (SavedLane *)s->save.p;
+        A.cont = (ContRec *)s->cont.p;
         A.id_end = total;
         A.iter_budget = 4096;
         A.max_steps = 1ull << 22;
@@ -399,6 +400,7 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
         (e = s->ctrl.ensure(CTRL_WORDS * 8)) != hipSuccess ||
         (e = s->tallies.ensure(s->tally_cap_words * 8)) != hipSuccess ||
         (e = s->save.ensure(s->lanes * sizeof(SavedLane))) != hipSuccess ||
+        (e = s->cont.ensure(s->lanes * sizeof(ContRec))) != hipSuccess ||
         (e = launch_clear_saved((SavedLane *)s->save.p, s->lanes, s->stream)) != hipSuccess ||
         (e = launch_fill_u64((uint64_t *)s->slot_done.p, (uint64_t)s->n_slots, 0, s->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(s->stream)) != hipSuccess) {
@@ -534,7 +536,7 @@ extern "C" octmps_status octmps_serve_close(octmps_server *s) {
     hipSetDevice(s->ctx->device);
     if (s->stream) hipStreamSynchronize(s->stream);
     for (DevBuf *b : {&s->regions, &s->sets, &s->slots, &s->seg_start, &s->seg_slot, &s->seg_poff, &s->tallies,
-                      &s->ctrl, &s->slot_done, &s->save})
+                      &s->ctrl, &s->slot_done, &s->save, &s->cont})
         b->release();
     if (s->e0) hipEventDestroy(s->e0);
     if (s->e1) hipEventDestroy(s->e1);

@@ -6,9 +6,11 @@
 // that many photon ids with one atomic and relaunches those lanes (ballot + mbcnt ranks).
 // Photon ids map to A-scans through a sorted segment table, and every photon derives its
 // three MWC streams from (seed, salt, photon index), so results do not depend on lane,
-// launch or slice. Transport is FP32; tallies are exact fixed-point integers (two 64-bit words
-// per value with a carry-free split, non-returning atomics; the reference's EMULATED_ATOMIC,
-// src/octmps_kernel.cuh:42-46, needed a returning add to propagate its carry).
+// launch or slice. Transport is FP32, compiled without FP contraction: every fused
+// multiply-add is written as fmaf, so each kernel variant rounds identically. Tallies are
+// exact fixed-point integers (two 64-bit words per value with a carry-free split,
+// non-returning atomics; the reference's EMULATED_ATOMIC, src/octmps_kernel.cuh:42-46,
+// needed a returning add to propagate its carry).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -52,20 +54,28 @@ __device__ __forceinline__ float rand_oc(uint64_t &x, uint32_t a) { return 1.0f 
 // division (__ddiv_rd), so no FP32 rounding mode is "the" reference one.
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Hardware square root (v_sqrt_f32, 1 ulp) instead of the correctly rounded sequence that
+// sqrtf compiles to; arguments here are clamped to >= 0.
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// -ln(u) for u in (0, 1] with the hardware base-2 logarithm (v_log_f32); u >= 2^-24 here,
+// so no denormal scaling is needed.
+__device__ __forceinline__ float neg_ln(float u) { return -0.69314718055994531f * __builtin_amdgcn_logf(u); }
+
 // Fresnel reflectance of FastReflectTransmit, src/octmps_kernel.cuh:608-645.
 __device__ __forceinline__ float fresnel_f(float ni, float nt, float ca1) {
     float ni_nt = ni * rcp(nt);
     bool normal = ca1 > kCosZeroF;
-    float sa1 = normal ? 0.0f : sqrtf(fmaxf(1.0f - ca1 * ca1, 0.0f));
+    float sa1 = normal ? 0.0f : fsqrt(fmaxf(fmaf(-ca1, ca1, 1.0f), 0.0f));
     float sa2 = fminf(ni_nt * sa1, 1.0f);
-    float ca2 = sqrtf(fmaxf(1.0f - sa2 * sa2, 0.0f));
+    float ca2 = fsqrt(fmaxf(fmaf(-sa2, sa2, 1.0f), 0.0f));
     float ca1ca2 = ca1 * ca2, sa1sa2 = sa1 * sa2;
     float sa1ca2 = normal ? 1.0f : sa1 * ca2;
     float ca1sa2 = normal ? ni_nt : ca1 * sa2;
     float cam = ca1ca2 + sa1sa2, sap = sa1ca2 + ca1sa2, sam = sa1ca2 - ca1sa2;
     float r = sam * rcp(sap * cam);
     r *= r;
-    r *= (ca1ca2 * ca1ca2 + sa1sa2 * sa1sa2);
+    r *= fmaf(ca1ca2, ca1ca2, sa1sa2 * sa1sa2);
     if (ca1 < kCos90F || sa2 == 1.0f) r = 1.0f;
     return r;
 }
@@ -73,8 +83,8 @@ __device__ __forceinline__ float fresnel_f(float ni, float nt, float ca1) {
 // SpinTheta, src/octmps_kernel.cuh:807-820 (hg_a = 1-g^2, hg_b = 1/(2g)).
 __device__ __forceinline__ float hg_cos(float g, float hg_a, float hg_b, float u_oc) {
     if (g == 0.0f) return 2.0f * u_oc - 1.0f;
-    float t = hg_a * rcp(1.0f - g + 2.0f * g * u_oc);
-    float c = (1.0f + g * g - t * t) * hg_b;
+    float t = hg_a * rcp(fmaf(2.0f * g, u_oc, 1.0f - g));
+    float c = fmaf(-t, t, fmaf(g, g, 1.0f)) * hg_b;
     return fminf(fmaxf(c, -1.0f), 1.0f);
 }
 
@@ -82,7 +92,7 @@ __device__ __forceinline__ float hg_cos(float g, float hg_a, float hg_b, float u
 __device__ __forceinline__ float bias_cos(float a, float inv_1ma, float inv_sq, float inv_2a,
                                           float opa2, float u_oc) {
     if (a == 0.0f) return u_oc;
-    float t = u_oc * inv_1ma + (1.0f - u_oc) * inv_sq;
+    float t = fmaf(u_oc, inv_1ma, (1.0f - u_oc) * inv_sq);
     float c = (opa2 - rcp(t * t)) * inv_2a;
     return fminf(fmaxf(c, -1.0f), 1.0f);
 }
@@ -93,11 +103,11 @@ __device__ __forceinline__ float bias_cos(float a, float inv_1ma, float inv_sq, 
 // copysign_mode: 0 -> uz = cost*sign(az) (Spin), 1 -> copysign(cost, az*cost) (SpinBias).
 __device__ __forceinline__ void rotate(float ax, float ay, float az, float cost, float r_psi,
                                        int copysign_mode, float &ux, float &uy, float &uz) {
-    float sint = sqrtf(fmaxf(1.0f - cost * cost, 0.0f));
+    float sint = fsqrt(fmaxf(fmaf(-cost, cost, 1.0f), 0.0f));
     // psi = 2 pi r: v_sin_f32 / v_cos_f32 take their argument in revolutions.
     const float sinp = __builtin_amdgcn_sinf(r_psi), cosp = __builtin_amdgcn_cosf(r_psi);
     float stcp = sint * cosp, stsp = sint * sinp;
-    float sp2 = ax * ax + ay * ay;
+    float sp2 = fmaf(ax, ax, ay * ay);
     float nx, ny, nz;
     if (sp2 <= kNormalSin2) {
         nx = stcp;
@@ -105,11 +115,11 @@ __device__ __forceinline__ void rotate(float ax, float ay, float az, float cost,
         nz = copysign_mode ? copysignf(cost, az * cost) : cost * (az >= 0.0f ? 1.0f : -1.0f);
     } else {
         float rs = rsqrtf(sp2);
-        nx = (stcp * ax * az - stsp * ay) * rs + ax * cost;
-        ny = (stcp * ay * az + stsp * ax) * rs + ay * cost;
-        nz = -stcp * (sp2 * rs) + az * cost;
+        nx = fmaf(fmaf(stcp * ax, az, -stsp * ay), rs, ax * cost);
+        ny = fmaf(fmaf(stcp * ay, az, stsp * ax), rs, ay * cost);
+        nz = fmaf(-stcp, sp2 * rs, az * cost);
     }
-    float rn = rsqrtf(nx * nx + ny * ny + nz * nz);
+    float rn = rsqrtf(fmaf(nx, nx, fmaf(ny, ny, nz * nz)));
     ux = nx * rn;
     uy = ny * rn;
     uz = nz * rn;
@@ -180,8 +190,10 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Six waves per SIMD (<= 80 VGPRs, no spills): the loop is bound by dependent gathers of
+// tet records, so more resident waves hide more of their latency.
 template <bool MULTI, bool LOG>
-__global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) transport_kernel(KArgs A) {
     const uint32_t lane = __lane_id();
     const uint64_t gl = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 
@@ -190,16 +202,14 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
     float lr = 1, lr_fst = 1, loc_fst = -1;
     int tet = 0, region = 1;
     uint32_t flags = 0, steps = 0;
-    // Continuation photon (photon_cont): the fields the restore can change.
-    float cx = 0, cy = 0, cz = 0, cux = 0, cuy = 0, cuz = 1, cw = 0, cop = 0, cmaxz = 0;
-    int ctet = 0, cregion = 1;
+    // The continuation photon (photon_cont) is this lane's ContRec.
+    ContRec &C = A.cont[gl];
     // Streams: General, ReflectT, Spin.
     uint64_t sG = 1, sR = 1, sS = 1;
     uint32_t aG = 2, aR = 2, aS = 2;
     // A-scan of the photon.
     uint32_t slot = 0;
     uint64_t id = 0;
-    float px = 0, py = 0, pz = 0;
     int set = 0;
     // Current region constants kept in registers.
     float rn = 1, rmuas = 1, muas = 1;
@@ -220,16 +230,14 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
             op = s.op; maxz = s.maxz; lr = s.lr; lr_fst = s.lr_fst; loc_fst = s.loc_fst;
             tet = s.tet; region = s.region; flags = s.flags; steps = s.steps;
             aG = s.aG; aR = s.aR; aS = s.aS; sG = s.sG; sR = s.sR; sS = s.sS;
-            cx = s.cx; cy = s.cy; cz = s.cz; cux = s.cux; cuy = s.cuy; cuz = s.cuz; cw = s.cw;
-            cop = s.cop; cmaxz = s.cmaxz; ctet = s.ctet; cregion = s.cregion;
             slot = s.slot; id = s.id; ndet = s.ndet; csum = s.csum;
-            const DevScan &sc = A.slots[slot];
-            px = sc.px; py = sc.py; pz = sc.pz;
-            set = MULTI ? sc.set : 0;
+            set = MULTI ? A.slots[slot].set : 0;
             load_region(region);
         }
     }
 
+    // One parameter set per launch unless MULTI: read it once (uniform, scalar registers).
+    const DevSet S0 = A.sets[0];
     bool drained = false;
     uint64_t wave_steps = 0, iters = 0;
     const uint32_t q = A.refill_q < 1u ? 1u : A.refill_q;
@@ -269,8 +277,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                         sG = derive_state(seeding_state(h, 0, b0), b0, aG);
                         sR = derive_state(seeding_state(h, 1, b1), b1, aR);
                         sS = derive_state(seeding_state(h, 2, b2), b2, aS);
-                        px = sc.px; py = sc.py; pz = sc.pz;
-                        x = px; y = py; z = pz;
+                        x = sc.px; y = sc.py; z = sc.pz;
                         ux = 0.0f; uy = 0.0f; uz = 1.0f;
                         w = sc.w0;
                         tau = 0.0f; op = 0.0f; maxz = 0.0f;
@@ -303,7 +310,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
         const bool ready = (flags & (F_ALIVE | F_PEND)) == F_ALIVE;
         wave_steps += (uint64_t)__popcll(__ballot(ready));
 
-        const DevSet &S = A.sets[set];
+        const DevSet &S = MULTI ? A.sets[set] : S0;
         bool finish = false, killed = false;
 
         // Roulette, src/octmps_kernel.cuh:1376-1392; returns true when the photon is done.
@@ -316,8 +323,14 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
             }
             if ((flags & F_FST) && S.type_bias != 3) {
                 const float lrt = lr_fst;
-                x = cx; y = cy; z = cz; ux = cux; uy = cuy; uz = cuz; w = cw;
-                op = cop; maxz = cmaxz; tet = ctet;
+                const float4 c0 = *reinterpret_cast<const float4 *>(&C.x);
+                const float4 c1 = *reinterpret_cast<const float4 *>(&C.ux);
+                const int4 c2 = *reinterpret_cast<const int4 *>(&C.maxz);
+                x = c0.x; y = c0.y; z = c0.z; w = c0.w;
+                ux = c1.x; uy = c1.y; uz = c1.z; op = c1.w;
+                maxz = __int_as_float(c2.x);
+                tet = c2.y;
+                const int cregion = c2.z;
                 tau = 0.0f;
                 flags = F_ALIVE | ((flags & F_CNBS) ? F_NBS : 0u);
                 loc_fst = -1.0f;
@@ -342,7 +355,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
         } else {
             // ---- ComputeStepSize, src/octmps_kernel.cuh:505-515
             float tau_d = tau;
-            if (tau_d == 0.0f) tau_d = -logf(rand_oc(sG, aG));
+            if (tau_d == 0.0f) tau_d = neg_ln(rand_oc(sG, aG));
             float s = tau_d * rmuas;
             tau = 0.0f;
             // ---- HitBoundary, src/octmps_kernel.cuh:517-572
@@ -350,27 +363,26 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
             const float4 p0 = P[0], p1 = P[1], p2 = P[2], p3 = P[3];
             const int4 L = *reinterpret_cast<const int4 *>(&A.topo[tet].link[0]);
             const uint32_t nregs = A.topo[tet].nbr_regions;
-            const float c0 = p0.x * ux + p0.y * uy + p0.z * uz;
-            const float c1 = p1.x * ux + p1.y * uy + p1.z * uz;
-            const float c2 = p2.x * ux + p2.y * uy + p2.z * uz;
-            const float c3 = p3.x * ux + p3.y * uy + p3.z * uz;
+            const float c0 = fmaf(p0.z, uz, fmaf(p0.y, uy, p0.x * ux));
+            const float c1 = fmaf(p1.z, uz, fmaf(p1.y, uy, p1.x * ux));
+            const float c2 = fmaf(p2.z, uz, fmaf(p2.y, uy, p2.x * ux));
+            const float c3 = fmaf(p3.z, uz, fmaf(p3.y, uy, p3.x * ux));
             float dmin = 1e10f, mincos = 0.0f;
             int imin = -1;
-#define OCTMPS_FACE(I, PL, C)                                            \
-    if (C < 0.0f) {                                                     \
-        const float perp = PL.x * x + PL.y * y + PL.z * z + PL.w;       \
-        const float d = -perp * rcp(C);                                 \
-        if (d < dmin) {                                                 \
-            dmin = d;                                                   \
-            imin = I;                                                   \
-            mincos = C;                                                 \
-        }                                                               \
-    }
-            OCTMPS_FACE(0, p0, c0)
-            OCTMPS_FACE(1, p1, c1)
-            OCTMPS_FACE(2, p2, c2)
-            OCTMPS_FACE(3, p3, c3)
-#undef OCTMPS_FACE
+            // Faces in order with a strict '<', as the reference; written with selects so
+            // every plane is loaded whole and the wave does not branch per face.
+            auto face = [&](int I, const float4 &PL, float C) {
+                const float perp = fmaf(PL.z, z, fmaf(PL.y, y, fmaf(PL.x, x, PL.w)));
+                const float d = -perp * rcp(C);
+                const bool take = (C < 0.0f) & (d < dmin);
+                dmin = take ? d : dmin;
+                imin = take ? I : imin;
+                mincos = take ? C : mincos;
+            };
+            face(0, p0, c0);
+            face(1, p1, c1);
+            face(2, p2, c2);
+            face(3, p3, c3);
             const bool hit = imin >= 0 && s > dmin;
             if (hit) {
                 // Glass (mus = 0) keeps the leftover unchanged, as (s - d) / DBL_MAX does
@@ -391,9 +403,13 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                 const int nxt = imin == 0 ? L.x : (imin == 1 ? L.y : (imin == 2 ? L.z : L.w));
                 const int nr = (int)((nregs >> (8 * imin)) & 0xffu);
                 const float ca1 = -mincos;
-                float nt;
+                // The empty asm keeps rn a register value: otherwise the three candidate
+                // loads are merged into one load through a phi of pointers, which puts rn
+                // in scratch memory and turns the load into a flat load.
+                float nt = rn;
+                asm("" : "+v"(nt));
                 if (nxt < 0) nt = S.n_ambient;
-                else nt = (nr == region) ? rn : A.regions[S.region_base + nr].n;
+                else if (nr != region) nt = A.regions[S.region_base + nr].n;
                 const float rnd = rand_co(sR, aR);
                 float R;
                 if (nxt >= 0 && nt == rn) R = (ca1 < kCos90F) ? 1.0f : 0.0f;  // sam == 0
@@ -405,8 +421,9 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                 if (nxt < 0 && R < 1.0f) {
                     if (uz < 0.0f) {
                         bool flag = false;
-                        const float dx = x - px, dy = y - py;
-                        if (dx * dx + dy * dy < S.radius2 && -uz > S.cos_angle) {
+                        const DevScan &SC = A.slots[slot];
+                        const float dx = x - SC.px, dy = y - SC.py;
+                        if (fmaf(dx, dx, dy * dy) < S.radius2 && -uz > S.cos_angle) {
                             const int tb = S.type_bias;
                             if (tb == 0 || tb == 3) flag = true;
                             if (((flags & F_FST) && tb == 37) || (loc_fst == maxz && tb != 37) ||
@@ -454,7 +471,7 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                 flags &= ~F_PEND;
                 // ---- Drop, src/octmps_kernel.cuh:584-588
                 const DevRegion RG = A.regions[S.region_base + region];
-                w -= w * RG.mua_muas;
+                w = fmaf(-w, RG.mua_muas, w);
                 const int tb = S.type_bias;
                 if (tb == 0) {
                     // Spin, src/octmps_kernel.cuh:822-879
@@ -467,8 +484,9 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                     bool first = false;
                     float atmp = 0.0f;
                     if (z > S.td_min && z < S.td_max && uz > 0.0f && !(flags & F_FST)) {
-                        cx = x; cy = y; cz = z; cux = ux; cuy = uy; cuz = uz; cw = w;
-                        cop = op; cmaxz = maxz; ctet = tet; cregion = region;
+                        *reinterpret_cast<float4 *>(&C.x) = make_float4(x, y, z, w);
+                        *reinterpret_cast<float4 *>(&C.ux) = make_float4(ux, uy, uz, op);
+                        *reinterpret_cast<int4 *>(&C.maxz) = make_int4(__float_as_int(maxz), tet, region, 0);
                         flags = (flags & ~F_CNBS) | ((flags & F_NBS) ? F_CNBS : 0u);
                         F = 1;
                         flags |= F_FST;
@@ -476,8 +494,9 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                         atmp = S.bias_a;
                         first = true;
                     }
-                    float vx = px - x, vy = py - y, vz = pz - z;
-                    const float len2 = vx * vx + vy * vy + vz * vz;
+                    const DevScan &SC = A.slots[slot];
+                    float vx = SC.px - x, vy = SC.py - y, vz = SC.pz - z;
+                    const float len2 = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
                     const float rl = rsqrtf(len2);
                     vx *= rl;
                     vy *= rl;
@@ -507,24 +526,24 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
                     cost = fminf(fmaxf(cost, -1.0f), 1.0f);
                     rotate(fx, fy, fz, cost, rand_co(sS, aS), 1, ux, uy, uz);
                     if (atmp != 0.0f) {
-                        const float costg = ux0 * ux + uy0 * uy + uz0 * uz;
+                        const float costg = fmaf(ux0, ux, fmaf(uy0, uy, uz0 * uz));
                         const float g = RG.g;
                         float factor;
                         if (F == 1) {
                             // Equation (8), src/octmps_kernel.cuh:1047-1050.
-                            const float t1 = S.opa2 - 2.0f * atmp * cost;
-                            const float t2 = 1.0f + g * g - 2.0f * g * costg;
-                            factor = RG.lr8 * (t1 * sqrtf(t1)) * rcp(t2 * sqrtf(t2));
+                            const float t1 = fmaf(-2.0f * atmp, cost, S.opa2);
+                            const float t2 = fmaf(-2.0f * g, costg, fmaf(g, g, 1.0f));
+                            factor = RG.lr8 * (t1 * fsqrt(t1)) * rcp(t2 * fsqrt(t2));
                         } else {
                             // Equation (9), src/octmps_kernel.cuh:1071-1078.
-                            const float costg1 = vx * ux + vy * uy + vz * uz;
+                            const float costg1 = fmaf(vx, ux, fmaf(vy, uy, vz * uz));
                             const float k1 = sel == 1 ? cost : costg1;
                             const float k2 = sel == 1 ? costg : cost;
-                            const float t1 = S.opa2 - 2.0f * atmp * k1;
-                            const float pdf1 = S.pdfb_k * rcp(t1 * sqrtf(t1));
-                            const float t2 = sqrtf(1.0f + g * g - 2.0f * g * k2);
+                            const float t1 = fmaf(-2.0f * atmp, k1, S.opa2);
+                            const float pdf1 = S.pdfb_k * rcp(t1 * fsqrt(t1));
+                            const float t2 = fsqrt(fmaf(-2.0f * g, k2, fmaf(g, g, 1.0f)));
                             const float pdf2 = RG.hg_a * rcp(2.0f * (t2 * t2 * t2));
-                            factor = pdf2 * rcp(S.p_add * pdf1 + (1.0f - S.p_add) * pdf2);
+                            factor = pdf2 * rcp(fmaf(S.p_add, pdf1, (1.0f - S.p_add) * pdf2));
                         }
                         lr *= factor;
                         if (first) lr_fst = lr;
@@ -556,12 +575,11 @@ __global__ void __launch_bounds__(256) transport_kernel(KArgs A) {
             s.op = op; s.maxz = maxz; s.lr = lr; s.lr_fst = lr_fst; s.loc_fst = loc_fst;
             s.tet = tet; s.region = region; s.steps = steps;
             s.aG = aG; s.aR = aR; s.aS = aS; s.sG = sG; s.sR = sR; s.sS = sS;
-            s.cx = cx; s.cy = cy; s.cz = cz; s.cux = cux; s.cuy = cuy; s.cuz = cuz; s.cw = cw;
-            s.cop = cop; s.cmaxz = cmaxz; s.ctet = ctet; s.cregion = cregion;
             s.slot = slot; s.id = id; s.ndet = ndet; s.csum = csum;
         }
     }
     if (lane == 0 && wave_steps) atomicAdd((unsigned long long *)&A.ctrl[CTRL_STEPS], (unsigned long long)wave_steps);
+    if (lane == 0 && iters) atomicAdd((unsigned long long *)&A.ctrl[CTRL_ITERS], (unsigned long long)iters);
 }
 
 __global__ void fill_u64_kernel(uint64_t *p, uint64_t n, uint64_t v) {

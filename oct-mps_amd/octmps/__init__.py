@@ -86,7 +86,7 @@ class RunStats(C.Structure):
         ("photons", C.c_uint64), ("photon_steps", C.c_uint64), ("killed", C.c_uint64),
         ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
         ("launches", C.c_uint32), ("block_size", C.c_uint32), ("grid_size", C.c_uint32),
-        ("refill_quantum", C.c_uint32),
+        ("refill_quantum", C.c_uint32), ("wave_iterations", C.c_uint64),
     ]
 
     def as_dict(self):
