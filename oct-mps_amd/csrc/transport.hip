@@ -216,8 +216,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     uint32_t ndet = 0;
     float csum = 0;
 
+    // One parameter set per launch unless MULTI: read it once (uniform, scalar registers),
+    // and keep its region table in LDS, since a photon changes region on a large share of
+    // its steps in layered tissue. MULTI reads the table of each lane's set from memory.
+    const DevSet S0 = A.sets[0];
+    __shared__ DevRegion lds_regions[MULTI ? 1 : kMaxRegions];
+    if (!MULTI) {
+        for (int i = threadIdx.x; i < kMaxRegions; i += blockDim.x) lds_regions[i] = A.regions[S0.region_base + i];
+        __syncthreads();
+    }
+    auto region_rec = [&](int r) -> const DevRegion & {
+        return MULTI ? A.regions[A.sets[set].region_base + r] : lds_regions[r];
+    };
     auto load_region = [&](int r) {
-        const DevRegion &R = A.regions[A.sets[set].region_base + r];
+        const DevRegion &R = region_rec(r);
         rn = R.n;
         rmuas = R.rmuas;
         muas = R.muas;
@@ -236,8 +248,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
         }
     }
 
-    // One parameter set per launch unless MULTI: read it once (uniform, scalar registers).
-    const DevSet S0 = A.sets[0];
     bool drained = false;
     uint64_t wave_steps = 0, iters = 0;
     const uint32_t q = A.refill_q < 1u ? 1u : A.refill_q;
@@ -338,7 +348,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                     region = cregion;
                     load_region(region);
                 }
-                const DevRegion &RC = A.regions[S.region_base + region];
+                const DevRegion &RC = region_rec(region);
                 const float cost = hg_cos(RC.g, RC.hg_a, RC.hg_b, rand_oc(sS, aS));
                 rotate(ux, uy, uz, cost, rand_co(sS, aS), 0, ux, uy, uz);
                 lr = (lrt < 1.0f) ? 1.0f - lrt : 1.0f;
@@ -409,7 +419,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                 float nt = rn;
                 asm("" : "+v"(nt));
                 if (nxt < 0) nt = S.n_ambient;
-                else if (nr != region) nt = A.regions[S.region_base + nr].n;
+                else if (nr != region) nt = region_rec(nr).n;
                 const float rnd = rand_co(sR, aR);
                 float R;
                 if (nxt >= 0 && nt == rn) R = (ca1 < kCos90F) ? 1.0f : 0.0f;  // sam == 0
@@ -470,7 +480,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             if (flags & F_PEND) {
                 flags &= ~F_PEND;
                 // ---- Drop, src/octmps_kernel.cuh:584-588
-                const DevRegion RG = A.regions[S.region_base + region];
+                const DevRegion RG = region_rec(region);
                 w = fmaf(-w, RG.mua_muas, w);
                 const int tb = S.type_bias;
                 if (tb == 0) {
