@@ -191,16 +191,25 @@ def main():
         idx = np.linspace(0, len(scans) - 1, a.cpu_scans).round().astype(int)
         ts = time.perf_counter()
         n = 0
+        subs, oracle_sum = [], 0.0
         for i in idx:
             s = scans[i]
             sub = O.ascan(s.x, s.y, s.z, s.root_tet, a.cpu_photons, s.stream_salt)
+            subs.append(sub)
             r = OR.run_ascan(soa, params, sub, seed, table, threads=threads)
             n += r["photons"]
+            oracle_sum += float(r["sum"].sum())
         cw = time.perf_counter() - ts
+        # The same photons (same streams) through the GPU kernel: error against the FP64 oracle.
+        tg, _ = ctx.run_bscan(mid, params, subs, seed)
+        gpu_sum = float(sum(O.decode_tally(x, params.num_depth_bins)["sum"].sum() for x in tg))
         cpu = {"value": n / cw, "unit": "photons/s", "cores": threads, "kind": "port",
                "sample": f"{a.cpu_scans} A-scans x {a.cpu_photons} photons of the same B-scan "
-                         f"(FP64 oracle, {threads} threads, {cw:.1f} s)"}
-        print(f"[bench] cpu oracle: {n} photons in {cw:.1f}s", file=sys.stderr, flush=True)
+                         f"(FP64 oracle, {threads} threads, {cw:.1f} s)",
+               "gpu_vs_oracle": {"photons": n, "reflectance_sum_gpu": gpu_sum, "reflectance_sum_oracle": oracle_sum,
+                                 "rel_err": abs(gpu_sum - oracle_sum) / oracle_sum if oracle_sum else None}}
+        print(f"[bench] cpu oracle: {n} photons in {cw:.1f}s; same photons on the GPU: summed reflectance "
+              f"{gpu_sum:.6e} vs oracle {oracle_sum:.6e}", file=sys.stderr, flush=True)
 
     serving = None
     if not a.no_serve:

@@ -240,6 +240,10 @@ typedef struct {
     int32_t use_graph;       /* capture the batched launch in a hipGraph */
     int32_t autotune;        /* pick block size / quantum on a calibration slice */
     int32_t scatter_quantum; /* lanes of a wave parked for scattering before it runs; 0 = default */
+    int32_t tally_copies;    /* tally privatisation depth: private copies of every A-scan's
+                                tally (workgroup w adds into copy w mod k), folded into one
+                                after the launch; 1..16, 0 = autotuned/default (1) */
+    int32_t reserved;        /* must be 0 */
     uint64_t max_steps_per_photon; /* 0 = default guard (1<<22) */
 } octmps_batch_opts;
 
@@ -266,8 +270,10 @@ octmps_status octmps_run_bscan_logged(octmps_ctx *ctx, int32_t mesh_id, const oc
                                       const octmps_batch_opts *opts, uint64_t *tallies_out,
                                       octmps_run_stats *stats, void *photon_log,
                                       uint64_t log_records);
-/* Autotune block size and refill quantum on a calibration slice of a B-scan; writes the
- * winner into *opts (block_size, refill_quantum) and its photons/s into *rate. */
+/* Autotune scatter quantum, block size, refill quantum and tally privatisation depth on a
+ * calibration slice of a B-scan (coordinate search, 16 launches); writes the winner into
+ * *opts (scatter_quantum, block_size, refill_quantum, tally_copies), makes it the context's
+ * default, and returns its photons/s in *rate. */
 octmps_status octmps_autotune(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p,
                               int32_t n_scans, const octmps_ascan *scans, uint64_t seed,
                               uint64_t calib_photons, octmps_batch_opts *opts, double *rate);

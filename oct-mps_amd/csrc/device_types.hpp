@@ -141,13 +141,18 @@ struct KArgs {
     uint32_t restore;           // 1: start from the saved lane states
     uint32_t exit_on_drain;     // slice mode: leave as soon as the ids run out
     uint32_t scatter_q;         // parked lanes per wave before the scattering code runs
-    uint32_t pad2;
+    uint32_t tally_copies;      // private tally copies; workgroup w adds into copy w % copies
+    uint64_t copy_stride;       // words between two copies of the same tally
 };
 
 // Launchers implemented in transport.hip.
 hipError_t launch_transport(const KArgs &a, int grid, int block, bool multi, bool log,
                             hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t stream);
+// Folds copies 1..copies-1 of n_scans tallies (each 4 + 12*bins words, copy c at
+// p + c*stride) into copy 0: sums, except the Max words, which take the maximum.
+hipError_t launch_fold_tallies(uint64_t *p, uint64_t n_scans, int bins, uint32_t copies, uint64_t stride,
+                               hipStream_t stream);
 hipError_t launch_clear_saved(SavedLane *p, uint64_t n, hipStream_t stream);
 int transport_blocks_per_cu(int block, bool multi, bool log);
 hipError_t launch_probe_rng(const uint32_t *mult, uint32_t n_triplets, uint64_t seed, uint64_t salt,

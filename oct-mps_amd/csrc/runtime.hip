@@ -145,6 +145,7 @@ octmps_status make_scan(octmps_ctx *ctx, const MeshEntry &me, const octmps_param
 int default_block(octmps_ctx *ctx) { return ctx->tuned_block > 0 ? ctx->tuned_block : 256; }
 int default_quantum(octmps_ctx *ctx) { return ctx->tuned_q > 0 ? ctx->tuned_q : 8; }
 int default_scatter(octmps_ctx *ctx) { return ctx->tuned_sq > 0 ? ctx->tuned_sq : 32; }
+int default_copies(octmps_ctx *ctx) { return ctx->tuned_copies > 0 ? ctx->tuned_copies : 1; }
 
 void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid) {
     int occ = transport_blocks_per_cu(block, multi, false);
@@ -196,6 +197,9 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     const int block = (opts && opts->block_size > 0) ? opts->block_size : default_block(ctx);
     if (block % 64 != 0 || block > 256) return fail(OCTMPS_E_INVALID, "block size must be 64, 128, 192 or 256");
     const int q = (opts && opts->refill_quantum > 0) ? opts->refill_quantum : default_quantum(ctx);
+    const int copies = (opts && opts->tally_copies > 0) ? opts->tally_copies : default_copies(ctx);
+    if (copies > 16 || (opts && opts->reserved != 0))
+        return fail(OCTMPS_E_INVALID, "tally_copies must be 1..16 and reserved 0");
     int grid = 0;
     grid_for(ctx, block, opts ? opts->blocks_per_cu : 0, false, grid);
     hipError_t e;
@@ -207,7 +211,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
         (e = W.seg_start.ensure(seg_start.size() * 8)) != hipSuccess ||
         (e = W.seg_slot.ensure(seg_slot.size() * 4)) != hipSuccess ||
         (e = W.seg_poff.ensure(seg_poff.size() * 8)) != hipSuccess ||
-        (e = W.tallies.ensure((size_t)n * words * 8)) != hipSuccess ||
+        (e = W.tallies.ensure((size_t)copies * n * words * 8)) != hipSuccess ||
         (e = W.ctrl.ensure(CTRL_WORDS * 8)) != hipSuccess ||
         (e = W.slot_done.ensure((size_t)n * 8)) != hipSuccess ||
         (e = W.cont.ensure((size_t)grid * block * sizeof(ContRec))) != hipSuccess)
@@ -243,17 +247,20 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     A.n_triplets = ctx->table->n_triplets;
     A.refill_q = (uint32_t)q;
     A.scatter_q = (uint32_t)((opts && opts->scatter_quantum > 0) ? opts->scatter_quantum : default_scatter(ctx));
+    A.tally_copies = (uint32_t)copies;
+    A.copy_stride = (uint64_t)n * words;
     const bool use_graph = opts && opts->use_graph && !plog_out;
     // Graph mode records the events around the graph launch (the fill kernels add a few us);
     // eager mode records them around the transport kernel alone.
     auto enqueue = [&](hipStream_t s2, bool timed) -> hipError_t {
         hipError_t e2;
-        if ((e2 = launch_fill_u64(A.tallies, (uint64_t)n * words, 0, s2)) != hipSuccess) return e2;
+        if ((e2 = launch_fill_u64(A.tallies, (uint64_t)copies * n * words, 0, s2)) != hipSuccess) return e2;
         if ((e2 = launch_fill_u64(A.ctrl, CTRL_WORDS, 0, s2)) != hipSuccess) return e2;
         if ((e2 = launch_fill_u64(A.slot_done, (uint64_t)n, 0, s2)) != hipSuccess) return e2;
         if (timed && (e2 = hipEventRecord(ctx->ev0, s2)) != hipSuccess) return e2;
         if ((e2 = launch_transport(A, grid, block, false, plog_out != nullptr, s2)) != hipSuccess) return e2;
-        return timed ? hipEventRecord(ctx->ev1, s2) : hipSuccess;
+        if (timed && (e2 = hipEventRecord(ctx->ev1, s2)) != hipSuccess) return e2;
+        return launch_fold_tallies(A.tallies, (uint64_t)n, B, A.tally_copies, A.copy_stride, s2);
     };
     if (use_graph) {
         GraphCache &G = ctx->graph;
@@ -485,13 +492,16 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     uint64_t per = std::max<uint64_t>(1, calib / (uint64_t)n);
     for (auto &s : cs) s.n_photons = std::min<uint64_t>(s.n_photons, per);
     std::vector<uint64_t> t((size_t)n * tally_words(p->num_depth_bins));
-    // Coordinate search: scatter quantum, then block size, then refill quantum, each with the
-    // best values found so far for the others (12 calibration launches).
+    // Coordinate search: scatter quantum, then block size, then refill quantum, then tally
+    // privatisation depth, each with the best values found so far for the others (16
+    // calibration launches).
     double best = -1;
     octmps_batch_opts win = *opts;
     win.block_size = 256;
     win.refill_quantum = 8;
     win.scatter_quantum = 32;
+    win.tally_copies = 1;
+    win.reserved = 0;
     win.use_graph = 0;
     auto trial = [&](octmps_batch_opts o) -> octmps_status {
         octmps_run_stats st;
@@ -520,12 +530,19 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
         o.refill_quantum = q;
         if ((s = trial(o)) != OCTMPS_OK) return s;
     }
+    for (int k : {2, 4, 8}) {
+        octmps_batch_opts o = win;
+        o.tally_copies = k;
+        if ((s = trial(o)) != OCTMPS_OK) return s;
+    }
     opts->block_size = win.block_size;
     opts->refill_quantum = win.refill_quantum;
     opts->scatter_quantum = win.scatter_quantum;
+    opts->tally_copies = win.tally_copies;
     c->tuned_block = win.block_size;
     c->tuned_q = win.refill_quantum;
     c->tuned_sq = win.scatter_quantum;
+    c->tuned_copies = win.tally_copies;
     if (rate) *rate = best;
     return OCTMPS_OK;
 }

@@ -73,7 +73,7 @@ struct octmps_ctx {
     std::mutex run_mu;
     octmps::WorkBuffers work;
     octmps::GraphCache graph;
-    int tuned_block = 0, tuned_q = 0, tuned_sq = 0;
+    int tuned_block = 0, tuned_q = 0, tuned_sq = 0, tuned_copies = 0;
 };
 
 namespace octmps {

@@ -160,7 +160,9 @@ __device__ __forceinline__ void add_split(uint64_t *lo_p, uint64_t *hi_p, uint64
 // Class I when MaxDepth > OP/2 - Lc/2.
 __device__ void tally_detection(const KArgs &A, const DevSet &S, uint32_t slot, float op, float maxz,
                                 float c) {
-    uint64_t *T = A.tallies + A.slots[slot].tally_off;
+    // Tally privatisation: workgroup w adds into copy w % copies (folded after the launch).
+    const uint64_t copy = A.tally_copies > 1 ? (uint64_t)(blockIdx.x % A.tally_copies) : 0;
+    uint64_t *T = A.tallies + A.slots[slot].tally_off + copy * A.copy_stride;
     atomicAdd((unsigned long long *)&T[0], 1ull);
     float depth = op * 0.5f;
     float q = (depth - S.shift) / S.bin_w;
@@ -667,6 +669,33 @@ hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t stre
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+__global__ void fold_tallies_kernel(uint64_t *p, uint64_t n_words, uint64_t words, uint64_t bins,
+                                    uint32_t copies, uint64_t stride) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t q = i % words;  // word within one A-scan's tally
+        const bool is_max = q >= 4 && (q - 4) % (6 * bins) >= 5 * bins;
+        uint64_t acc = p[i];
+        for (uint32_t c = 1; c < copies; ++c) {
+            const uint64_t v = p[i + c * stride];
+            acc = is_max ? (v > acc ? v : acc) : acc + v;  // Max words: non-negative IEEE bits
+        }
+        p[i] = acc;
+    }
+}
+
+hipError_t launch_fold_tallies(uint64_t *p, uint64_t n_scans, int bins, uint32_t copies, uint64_t stride,
+                               hipStream_t stream) {
+    if (copies <= 1 || n_scans == 0) return hipSuccess;
+    const uint64_t words = 4 + 12 * (uint64_t)bins;
+    const uint64_t n = n_scans * words;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fold_tallies_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, words,
+                       (uint64_t)bins, copies, stride);
     return hipGetLastError();
 }
 

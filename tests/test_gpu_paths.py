@@ -35,6 +35,21 @@ def test_uneven_budgets_batched_equals_plain(ctx, small_meshes):
         assert np.array_equal(ta, tb[i])
 
 
+@pytest.mark.parametrize("copies,graph", [(2, 0), (5, 1), (16, 1)])
+def test_tally_privatisation_depth_never_changes_results(ctx, small_meshes, copies, graph):
+    # k private tally copies, folded after the launch: integer sums and maxima are exact,
+    # so every depth gives the same words as one copy.
+    m, mid, p, scans, seed = small_meshes["ellipsoid"]
+    one = O.BatchOpts()
+    one.tally_copies = 1
+    ref, _ = ctx.run_bscan(mid, p, scans, seed, opts=one)
+    o = O.BatchOpts()
+    o.tally_copies, o.use_graph = copies, graph
+    for _ in range(2):
+        got, _ = ctx.run_bscan(mid, p, scans, seed, opts=o)
+        assert np.array_equal(ref, got)
+
+
 def test_served_results_equal_batched_alone_and_in_a_mix(ctx, small_meshes):
     ref = {k: ctx.run_bscan(v[1], v[2], v[3], v[4])[0] for k, v in small_meshes.items()}
     srv = O.Server(ctx, slice_photons=4096)
@@ -117,6 +132,7 @@ def test_autotune_picks_a_valid_configuration(ctx, small_meshes):
     _, mid, p, s, seed = small_meshes["slab"]
     o, rate = ctx.autotune(mid, p, s, seed, calib=4000)
     assert o.block_size in (64, 128, 256) and o.refill_quantum in (1, 4, 8, 16, 32) and rate > 0
+    assert o.scatter_quantum in (1, 16, 32, 48) and o.tally_copies in (1, 2, 4, 8)
     t1 = ctx.run_bscan(mid, p, s, seed)[0]
     t2 = ctx.run_bscan(mid, p, s, seed, opts=o)[0]
     assert np.array_equal(t1, t2)  # launch shape never changes results
