@@ -48,6 +48,11 @@ def parse():
                     help="run a plain copy kernel of this size first (byte-counter calibration)")
     ap.add_argument("--plain-every", type=int, default=8,
                     help="time the plain path on every k-th A-scan of the B-scan")
+    ap.add_argument("--workload", choices=("slab4", "ellipsoid", "bump"), default="slab4",
+                    help="slab4: the headline B-scan; ellipsoid: 512 positions with log-uniform budgets "
+                         "in [1e5, 1e7] (seed 7); bump: 6.3M-tet bump mesh, 1024 A-scans x 1e6 photons")
+    ap.add_argument("--stride", type=int, default=1,
+                    help="keep every k-th A-scan of the workload (a bounded sample of a long B-scan)")
     return ap.parse_args()
 
 
@@ -117,7 +122,19 @@ def main():
     from octmps import configs
 
     t0 = time.time()
-    mesh, params, scans, seed = configs.slab4(n_scans=a.scans, photons=a.photons)
+    if a.workload == "slab4":
+        mesh, params, scans, seed = configs.slab4(n_scans=a.scans, photons=a.photons)
+        desc = (f"slab4 B-scan: 4x4x2 mm four-layer slab, 64x64x32 cells x 6 tets ({mesh.num_tets} tets), "
+                f"{{n}} A-scans x {a.photons} photons at 20 um, 512 bins to 1.5 mm, TypeBias 37")
+    elif a.workload == "ellipsoid":
+        mesh, params, scans, seed = configs.ellipsoid()
+        desc = (f"ellipsoid + two spheres: 4x4x2 mm, 64x64x32 cells x 6 tets ({mesh.num_tets} tets), "
+                f"{{n}} of 512 A-scan positions with log-uniform budgets in [1e5, 1e7] (seed 7), 512 bins")
+    else:
+        mesh, params, scans, seed = configs.bump()
+        desc = (f"bump: 6x6x3 mm, 128x128x64 cells x 6 tets ({mesh.num_tets} tets), {{n}} of 1024 A-scans x "
+                f"1e6 photons, probes projected onto the bump, 1024 bins to 2 mm")
+    scans = scans[::max(1, a.stride)]
     print(f"[bench] mesh {mesh.num_tets} tets built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     ctx = O.Context(0)
     mid = ctx.upload(mesh)
@@ -231,9 +248,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"slab4 B-scan: 4x4x2 mm four-layer slab, 64x64x32 cells x 6 tets "
-                               f"({mesh.num_tets} tets), {len(scans)} A-scans x {a.photons} photons at 20 um, "
-                               f"512 bins to 1.5 mm, TypeBias 37",
+        "config": {"workload": desc.format(n=len(scans)),
                    "photons": total_photons, "path": "batched persistent kernel, hipGraph launch"},
         "batched": {"photons_per_s": value, "kernel_ms": kms, "photon_steps": steps,
                     "steps_per_photon": steps_per_launch / total_photons,
