@@ -70,6 +70,24 @@ def test_sphere_parity_paired(ctx, table):
     assert c["R_rel"] < 0.005
 
 
+@pytest.mark.parametrize("name", ["slab4", "ellipsoid"])
+def test_layered_and_inclusion_parity_paired(ctx, table, name):
+    # Four index-mismatched layers (Fresnel at every layer crossing) and an ellipsoid with two
+    # sphere inclusions (three regions, curved interfaces), on coarser meshes of the same
+    # geometry so the FP64 oracle finishes in seconds. One A-scan at the centre.
+    n = 12000 if name == "slab4" else 40000
+    if name == "slab4":
+        mesh, params, scans, seed = configs.slab4(cells=(32, 32, 16), n_scans=1, photons=n, bins=256)
+    else:
+        mesh, params, scans, seed = configs.ellipsoid(cells=(32, 32, 16), n_scans=1, budgets=[n], bins=256)
+    gpu, st, plog, ref = _run_pair(ctx, table, mesh, params, scans[0], seed, threads=16)
+    assert st.photons == n and st.killed == 0 and ref["killed"] == 0
+    c = compare(gpu, ref, n)
+    print(name, c)
+    assert c["frac_3sigma"] <= 0.01 and c["max_abs_z"] < 5
+    assert c["R_rel"] < 0.005
+
+
 def test_glass_slab_known_answer(ctx):
     # One glass region (mus = 0: no scattering) with n = 1.5 under air. Every photon travels
     # straight down and up; at the first return it is detected with weight
