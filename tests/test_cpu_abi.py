@@ -43,3 +43,33 @@ def test_no_device_is_an_error_not_a_crash():
         assert e.status == 8
     else:
         raise AssertionError("context created without a device")
+
+
+# ctypes mirror -> C struct name of include/octmps.h
+STRUCTS = {
+    "Region": "octmps_region", "Params": "octmps_params", "GridSpec": "octmps_grid_spec",
+    "AScan": "octmps_ascan", "RunStats": "octmps_run_stats", "BatchOpts": "octmps_batch_opts",
+    "ServeOpts": "octmps_serve_opts", "Request": "octmps_request", "RequestInfo": "octmps_request_info",
+}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    # Compile a probe against the header with the host C compiler and compare the size and
+    # every field offset with the ctypes mirror used by the binding.
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "octmps.h"', "int main(void) {"]
+    for py, c in STRUCTS.items():
+        lines.append(f'printf("{py} size %zu\\n", sizeof({c}));')
+        for f, _ in getattr(O, py)._fields_:
+            lines.append(f'printf("{py} {f} %zu\\n", offsetof({c}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        py, field, value = line.split()
+        cls = getattr(O, py)
+        want = ctypes.sizeof(cls) if field == "size" else getattr(cls, field).offset
+        assert int(value) == want, (py, field, int(value), want)

@@ -50,6 +50,17 @@ def test_tally_privatisation_depth_never_changes_results(ctx, small_meshes, copi
         assert np.array_equal(ref, got)
 
 
+@pytest.mark.parametrize("field,value", [("tally_copies", 17), ("reserved", 1), ("block_size", 96),
+                                         ("block_size", 512)])
+def test_invalid_launch_options_are_rejected(ctx, small_meshes, field, value):
+    m, mid, p, scans, seed = small_meshes["ellipsoid"]
+    o = O.BatchOpts()
+    setattr(o, field, value)
+    with pytest.raises(O.OctmpsError) as e:
+        ctx.run_bscan(mid, p, scans, seed, opts=o)
+    assert e.value.status == 1
+
+
 def test_served_results_equal_batched_alone_and_in_a_mix(ctx, small_meshes):
     ref = {k: ctx.run_bscan(v[1], v[2], v[3], v[4])[0] for k, v in small_meshes.items()}
     srv = O.Server(ctx, slice_photons=4096)
