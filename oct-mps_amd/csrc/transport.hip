@@ -252,6 +252,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
 
     bool drained = false;
     uint64_t wave_steps = 0, iters = 0;
+    // Step guard as a 32-bit count (the per-photon counter is 32-bit).
+    const uint32_t max_steps = A.max_steps > 0xffffffffull ? 0xffffffffu : (uint32_t)A.max_steps;
     const uint32_t q = A.refill_q < 1u ? 1u : A.refill_q;
 
     for (;;) {
@@ -361,7 +363,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
 
         if (!ready) {
             // parked or dead lane: nothing to do in the crossing phase
-        } else if (++steps > A.max_steps) {
+        } else if (++steps > max_steps) {
             finish = true;
             killed = true;
         } else {
