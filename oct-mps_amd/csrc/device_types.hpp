@@ -58,7 +58,8 @@ struct DevSet {
     float inv_2a;        // 1/(2a)
     float opa2;          // 1 + a^2
     float pdfb_k;        // sqrt(1+a^2) a (1-a) / (sqrt(1+a^2)-1+a)
-    int32_t region_base; // index of region 0 of this set in the region table
+    int32_t region_base; // index of region 0 of this set in the region table: always
+                         // set * kMaxRegions (the kernel computes it from the set index)
     int32_t pad;
 };
 static_assert(sizeof(DevSet) == 80, "DevSet must be 80 bytes");
@@ -143,6 +144,8 @@ struct KArgs {
     uint32_t scatter_q;         // parked lanes per wave before the scattering code runs
     uint32_t tally_copies;      // private tally copies; workgroup w adds into copy w % copies
     uint64_t copy_stride;       // words between two copies of the same tally
+    uint64_t slice_ticks;       // slice mode: wall-clock length of a slice (device wall clock
+                                // ticks, 0 = none); every wave leaves at about the same time
 };
 
 // Launchers implemented in transport.hip.

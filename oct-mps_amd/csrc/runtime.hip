@@ -339,6 +339,8 @@ extern "C" octmps_status octmps_ctx_create(int32_t device, octmps_ctx **out) {
     }
     c->n_cu = pr.multiProcessorCount;
     c->hbm_total = pr.totalGlobalMem;
+    if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess)
+        c->wall_khz = 0;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
         (e = hipMalloc(&c->d_mult, t->mult.size() * 4)) != hipSuccess ||

@@ -58,6 +58,7 @@ octmps_status make_set(const octmps_params *p, int max_region, int region_base, 
 struct octmps_ctx {
     int device = 0;
     int n_cu = 0;
+    int wall_khz = 0;  // rate of the device wall clock read by s_memrealtime
     uint64_t hbm_total = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

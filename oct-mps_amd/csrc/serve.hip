@@ -290,7 +290,10 @@ void scheduler(octmps_server *s) {
         A.save = (SavedLane *)s->save.p;
         A.cont = (ContRec *)s->cont.p;
         A.id_end = total;
-        A.iter_budget = 4096;
+        // A slice lasts ~20 ms of device wall clock (iteration cap as a backstop).
+        A.iter_budget = 1u << 16;
+        A.slice_ticks = ctx->wall_khz > 0 ? (uint64_t)ctx->wall_khz * 20 : 0;
+        if (!A.slice_ticks) A.iter_budget = 8192;
         A.max_steps = 1ull << 22;
         A.n_seg = (int32_t)seg_slot.size();
         A.n_triplets = ctx->table->n_triplets;

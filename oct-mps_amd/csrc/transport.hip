@@ -228,7 +228,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
         __syncthreads();
     }
     auto region_rec = [&](int r) -> const DevRegion & {
-        return MULTI ? A.regions[A.sets[set].region_base + r] : lds_regions[r];
+        // The region table of set s starts at s * kMaxRegions (DevSet::region_base).
+        return MULTI ? A.regions[set * kMaxRegions + r] : lds_regions[r];
     };
     auto load_region = [&](int r) {
         const DevRegion &R = region_rec(r);
@@ -237,7 +238,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
         muas = R.muas;
     };
 
-    if (A.restore) {
+    // Slices (save/restore of live photons between launches) serve the request server, which
+    // always launches the MULTI variant; the single-set variants run in drain mode only.
+    constexpr bool SLICES = MULTI;
+    if (SLICES && A.restore) {
         const SavedLane &s = A.save[gl];
         if (s.flags & F_ALIVE) {
             x = s.x; y = s.y; z = s.z; ux = s.ux; uy = s.uy; uz = s.uz; w = s.w; tau = s.tau;
@@ -252,6 +256,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
 
     bool drained = false;
     uint64_t wave_steps = 0, iters = 0;
+    const uint64_t t_start = (SLICES && A.slice_ticks) ? __builtin_amdgcn_s_memrealtime() : 0;
     // Step guard as a 32-bit count (the per-photon counter is 32-bit).
     const uint32_t max_steps = A.max_steps > 0xffffffffull ? 0xffffffffu : (uint32_t)A.max_steps;
     const uint32_t q = A.refill_q < 1u ? 1u : A.refill_q;
@@ -315,7 +320,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             if (drained) break;
             continue;
         }
-        if (A.slice_mode && ((drained && A.exit_on_drain) || (A.iter_budget && iters >= A.iter_budget))) break;
+        if (SLICES && A.slice_mode &&
+            ((drained && A.exit_on_drain) || (A.iter_budget && iters >= A.iter_budget)))
+            break;
+        // Slices end on the device wall clock (checked every 32 iterations), so all waves of
+        // the grid leave together instead of the slice lasting as long as its slowest wave.
+        if (SLICES && A.slice_mode && A.slice_ticks && (iters & 31u) == 0 &&
+            __builtin_amdgcn_s_memrealtime() - t_start > A.slice_ticks)
+            break;
         ++iters;
         // A lane whose step ended inside the tet (absorb + scatter) parks with F_PEND until
         // enough lanes of the wave need the scattering code (scatter quantum), while the
@@ -581,7 +593,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
         }
     }
 
-    if (A.slice_mode) {
+    if (SLICES && A.slice_mode) {
         SavedLane &s = A.save[gl];
         s.flags = flags;
         if (flags & F_ALIVE) {
