@@ -88,6 +88,26 @@ def test_layered_and_inclusion_parity_paired(ctx, table, name):
     assert c["R_rel"] < 0.005
 
 
+@pytest.mark.parametrize("type_bias", [0, 3])
+def test_unbiased_and_no_spin_parity_paired(ctx, table, type_bias):
+    # TypeBias 0 scatters with Spin (Henyey-Greenstein only) and accepts every photon that
+    # reaches the detector; TypeBias 3 does not turn photons at interaction sites (the
+    # reference's switch has no case for it) and never restores a continuation photon.
+    # A wide detector (0.5 mm, 30 deg) gives enough detections without the bias.
+    n = 12000
+    mesh, params, scans, seed = configs.sphere(cells=16, photons=n)
+    params.type_bias = type_bias
+    params.max_collecting_radius = 0.05
+    params.max_collecting_angle_deg = 30.0
+    gpu, st, plog, ref = _run_pair(ctx, table, mesh, params, scans[0], seed, threads=16)
+    assert st.photons == n and st.killed == 0 and ref["killed"] == 0
+    assert ref["num_filtered"] > 100
+    c = compare(gpu, ref, n)
+    print(type_bias, c)
+    assert c["frac_3sigma"] <= 0.01 and c["max_abs_z"] < 5
+    assert c["R_rel"] < 0.005
+
+
 def test_glass_slab_known_answer(ctx):
     # One glass region (mus = 0: no scattering) with n = 1.5 under air. Every photon travels
     # straight down and up; at the first return it is detected with weight
