@@ -24,6 +24,16 @@
 //   * a photon exceeding max_steps transport steps is stopped and counted (no guard exists
 //     in the reference);
 //   * a negative bin position saturates to 0 like CUDA's double-to-unsigned conversion.
+//
+// Pins (tests/test_cpu_rng.py, tests/test_cpu_oracle_pins.py, tests/test_cpu_formats.py):
+// MWC draws and stream seeding against big-integer arithmetic; Fresnel against the textbook
+// formula; the HG and biased samplers as inverse CDFs; equations (8) and (9) through their
+// sphere integrals; Spin's mean direction; HitBoundary against a barycentric brute force;
+// the whole loop against a glass-slab known answer. Parity unpinned (the reference ships no
+// golden outputs and its kernel cannot run here): SpinBias's flag transitions
+// (src/octmps_kernel.cuh:918-968), the detector filter rule (:674-681) and the
+// continuation-photon restore (:1376-1387) are checked only through those composed
+// identities and the known answer, not against reference output.
 #include <atomic>
 #include <cfloat>
 #include <cmath>
