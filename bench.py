@@ -99,11 +99,20 @@ def run_serving(O, configs, ctx, a, log):
         first = info.t_submit_ms if first is None else min(first, info.t_submit_ms)
         last = info.t_done_ms if last is None else max(last, info.t_done_ms)
     st = srv.stats()
+    # Latency floor: one small request (1 A-scan x 1e4 photons, sphere) on the idle server,
+    # five times.
+    mesh, mid, p, probe = meshes[0]
+    floor = []
+    for k in range(5):
+        small = [O.ascan(probe.x, probe.y, probe.z, probe.root_tet, 10000, salt=999000 + k)]
+        info, _ = srv.wait(srv.submit(mid, p, small, 1), -1.0)
+        floor.append(info.t_done_ms - info.t_submit_ms)
     srv.close()
     lat = np.array(lat)
     span_s = (last - first) * 1e-3
     return {"requests": len(ids), "rate_req_s": a.serve_rate, "arrival_window_s": a.serve_window,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "idle_small_request_ms": float(np.median(floor)),
             "photons": photons, "photons_per_s": photons / span_s, "span_s": span_s,
             "slices": st["slices"], "kernel_busy_frac": st["kernel_ms"] * 1e-3 / span_s,
             "meshes": "sphere 196,608 + slab4 786,432 + ellipsoid 786,432 + "

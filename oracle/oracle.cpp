@@ -29,7 +29,8 @@
 // MWC draws and stream seeding against big-integer arithmetic; Fresnel against the textbook
 // formula; the HG and biased samplers as inverse CDFs; equations (8) and (9) through their
 // sphere integrals; Spin's mean direction; HitBoundary against a barycentric brute force;
-// the whole loop against a glass-slab known answer. Parity unpinned (the reference ships no
+// the detector tally's window, counters and class rule on enumerated edge cases; the whole
+// loop against a glass-slab known answer. Parity unpinned (the reference ships no
 // golden outputs and its kernel cannot run here): SpinBias's flag transitions
 // (src/octmps_kernel.cuh:918-968), the detector filter rule (:674-681) and the
 // continuation-photon restore (:1376-1387) are checked only through those composed
@@ -794,6 +795,29 @@ void oracle_hit_boundary(const double *planes16, const double *pos, const double
     out[2] = p.s;
     out[3] = p.sleft;
     out[4] = p.MinCos;
+}
+
+// One detection through the detector tally (src/octmps_kernel.cuh:684-741) of an empty
+// tally: sums[2][B] (Class I, II), counts[2][B], counters[3] = filtered, Class I, Class II.
+void oracle_tally_detection(const octmps_params *prm, double op, double maxz, double w, double rF, double lr,
+                            double *sums, uint64_t *counts, uint64_t *counters) {
+    Sim s;
+    make_sim(prm, s);
+    Photon p;
+    std::memset(&p, 0, sizeof p);
+    p.OpticalPath = op;
+    p.MaxDepth = maxz;
+    p.w = w;
+    p.LikelihoodRatio = lr;
+    Tally T(prm->num_depth_bins);
+    tally_detection(s, p, rF, T, nullptr);
+    for (int k = 0; k < 2 * T.B; ++k) {
+        sums[k] = T.sum[k];
+        counts[k] = T.count[k];
+    }
+    counters[0] = T.nfilt;
+    counters[1] = T.nc1;
+    counters[2] = T.nc2;
 }
 
 // n independent SpinBias calls on copies of one photon state; Spin stream per call from

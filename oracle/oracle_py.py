@@ -46,6 +46,9 @@ def lib():
         L.oracle_spin_bias_batch.restype = None
         L.oracle_spin_bias_batch.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_uint64, u32p,
                                              C.c_uint32, C.c_int, dp]
+        L.oracle_tally_detection.restype = None
+        L.oracle_tally_detection.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                             C.c_double, dp, up, up]
         _lib = L
     return _lib
 
@@ -147,6 +150,17 @@ def hit_boundary(planes16, pos, direction, s, mut):
                               _dp(np.ascontiguousarray(pos, np.float64)),
                               _dp(np.ascontiguousarray(direction, np.float64)), s, mut, _dp(out))
     return out
+
+
+def tally_detection(params, op, maxz, w, r_fresnel, lr):
+    """One detection into an empty tally: (sums[2, B], counts[2, B], counters[3])."""
+    B = params.num_depth_bins
+    sums = np.zeros(2 * B, np.float64)
+    counts = np.zeros(2 * B, np.uint64)
+    counters = np.zeros(3, np.uint64)
+    lib().oracle_tally_detection(C.addressof(params), op, maxz, w, r_fresnel, lr, _dp(sums), _up(counts),
+                                 _up(counters))
+    return sums.reshape(2, B), counts.reshape(2, B), counters
 
 
 def spin_bias_batch(params, state, probe, seed, table, n):

@@ -278,3 +278,52 @@ def test_type_bias_zero_tallies_every_aperture_hit(table):
     r = OR.run_ascan(mesh.soa(), params, O.ascan(0.0013, 0.0007, z, root, 2000), 1, table, threads=4)
     assert r["killed"] == 0 and r["num_filtered"] > 0
     assert r["num_class1"] + r["num_class2"] <= r["num_filtered"]
+
+
+def _bin_params(bins=20, shift=0.0):
+    p = O.make_params([(1.4, 1.0, 100.0, 0.9)], num_bins=bins, depth=bins * 2.0 ** -10)
+    p.optical_depth_shift = shift
+    return p
+
+
+# (OP/2 in bin units, bins receiving the contribution, Class counter bumped)
+# from the reference's window [VP-3, VP+3) clipped to [0, B) and its `ii == VP` counter
+# (src/octmps_kernel.cuh:684-741), B = 20, bin width 2^-10 cm.
+BIN_CASES = [
+    (10.5, [7, 8, 9, 10, 11, 12], True),
+    (10.0, [7, 8, 9, 10, 11, 12], True),      # exact edge: VP = 10, not 9
+    (0.5, [0, 1, 2], True),                    # window clipped at 0
+    (19.5, [16, 17, 18, 19], True),            # clipped at B
+    (21.0, [18, 19], False),                   # VP = 21 outside: bins 18, 19 only, no class count
+    (30.0, [], False),                         # nothing in range, still one filtered detection
+]
+
+
+@pytest.mark.parametrize("half_op,bins,counted", BIN_CASES)
+def test_tally_window_and_counters(half_op, bins, counted):
+    p = _bin_params()
+    w, rF, lr = 0.5, 0.25, 0.75
+    c = w * (1 - rF) * lr
+    op = 2 * half_op * 2.0 ** -10
+    maxz = 1.0  # deep: Class I
+    sums, counts, ctr = OR.tally_detection(p, op, maxz, w, rF, lr)
+    want = np.zeros(20)
+    want[bins] = c
+    assert np.array_equal(sums[0], want) and not sums[1].any()
+    assert np.array_equal(counts[0], (want > 0).astype(np.uint64))
+    assert list(ctr) == [1, 1 if counted else 0, 0]
+
+
+def test_tally_class_rule_is_strict_and_negative_positions_saturate():
+    p = _bin_params()
+    lc = p.coherence_length_source  # 6 * 2^-10: every quantity below is exact in binary
+    op = 21 * 2.0 ** -10            # OP/2 = 10.5 bins
+    edge = op / 2 - lc / 2          # Class I needs MaxDepth > OP/2 - Lc/2
+    _, counts, ctr = OR.tally_detection(p, op, edge, 1.0, 0.0, 1.0)
+    assert counts[1].sum() == 6 and counts[0].sum() == 0 and list(ctr) == [1, 0, 1]
+    _, counts, ctr = OR.tally_detection(p, op, np.nextafter(edge, 1.0), 1.0, 0.0, 1.0)
+    assert counts[0].sum() == 6 and list(ctr) == [1, 1, 0]
+    # OP/2 below the shift: the position saturates to bin 0 (window 0..2).
+    q = _bin_params(shift=5 * 2.0 ** -10)
+    _, counts, ctr = OR.tally_detection(q, 4 * 2.0 ** -10, 1.0, 1.0, 0.0, 1.0)
+    assert list(np.nonzero(counts[0])[0]) == [0, 1, 2] and list(ctr) == [1, 1, 0]
