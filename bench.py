@@ -226,12 +226,25 @@ def main():
             n += r["photons"]
             oracle_sum += float(r["sum"].sum())
         cw = time.perf_counter() - ts
+        # Single-thread rate on the first sampled A-scan (a quarter of the photons).
+        s0 = scans[idx[0]]
+        one = O.ascan(s0.x, s0.y, s0.z, s0.root_tet, max(1, a.cpu_photons // 4), s0.stream_salt)
+        ts = time.perf_counter()
+        r1 = OR.run_ascan(soa, params, one, seed, table, threads=1)
+        single = r1["photons"] / (time.perf_counter() - ts)
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+        except OSError:
+            pass
         # The same photons (same streams) through the GPU kernel: error against the FP64 oracle.
         tg, _ = ctx.run_bscan(mid, params, subs, seed)
         gpu_sum = float(sum(O.decode_tally(x, params.num_depth_bins)["sum"].sum() for x in tg))
         cpu = {"value": n / cw, "unit": "photons/s", "cores": threads, "kind": "port",
                "sample": f"{a.cpu_scans} A-scans x {a.cpu_photons} photons of the same B-scan "
                          f"(FP64 oracle, {threads} threads, {cw:.1f} s)",
+               "single_thread_value": single, "cpu_model": model, "nproc": os.cpu_count(),
                "gpu_vs_oracle": {"photons": n, "reflectance_sum_gpu": gpu_sum, "reflectance_sum_oracle": oracle_sum,
                                  "rel_err": abs(gpu_sum - oracle_sum) / oracle_sum if oracle_sum else None}}
         print(f"[bench] cpu oracle: {n} photons in {cw:.1f}s; same photons on the GPU: summed reflectance "

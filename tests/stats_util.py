@@ -10,6 +10,25 @@ def bin_stats(total, total_sq, n):
     return mean, np.sqrt(np.maximum(var, 0.0))
 
 
+def filtered(t, max_rel=0.01):
+    """The *_ScattFilt outputs' rule (src/octmps_io.cpp:359-420): a bin whose largest single
+    contribution exceeds max_rel of its sum drops that contribution."""
+    s, q, mx = t["sum"], t["sumsq"], t["max"]
+    cut = mx > max_rel * s
+    return {"sum": np.where(cut, s - mx, s), "sumsq": np.where(cut, q - mx * mx, q)}
+
+
+def chi2_decimated(gpu, ref, n_gpu, n_ref, stride=6):
+    """Chi^2 of independent runs over every `stride`-th bin (a detection spans 6 consecutive
+    bins, so bins 6 apart share none) of both classes; returns (chi2, ndf)."""
+    mg, sg = bin_stats(gpu["sum"], gpu["sumsq"], n_gpu)
+    mr, sr = bin_stats(ref["sum"], ref["sumsq"], n_ref)
+    mg, sg, mr, sr = (a[:, ::stride] for a in (mg, sg, mr, sr))
+    sig2 = sg ** 2 + sr ** 2
+    pop = sig2 > 0
+    return float((((mg - mr) ** 2)[pop] / sig2[pop]).sum()), int(pop.sum())
+
+
 def compare(gpu, ref, n):
     """Per-bin z scores over both classes, plus total reflectance of each side.
 
@@ -31,4 +50,9 @@ def compare(gpu, ref, n):
         "R_gpu": float(rg),
         "R_ref": float(rr),
         "R_rel": float(abs(rg - rr) / rr) if rr else float(abs(rg - rr)),
+        "Rfilt_rel": _rel(filtered(gpu)["sum"].sum(), filtered(ref)["sum"].sum()),
     }
+
+
+def _rel(a, b):
+    return float(abs(a - b) / b) if b else float(abs(a - b))
