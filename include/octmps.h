@@ -271,7 +271,7 @@ octmps_status octmps_run_bscan_logged(octmps_ctx *ctx, int32_t mesh_id, const oc
                                       octmps_run_stats *stats, void *photon_log,
                                       uint64_t log_records);
 /* Autotune scatter quantum, block size, refill quantum and tally privatisation depth on a
- * calibration slice of a B-scan (coordinate search, 16 launches); writes the winner into
+ * calibration slice of a B-scan (coordinate search, 14 launches); writes the winner into
  * *opts (scatter_quantum, block_size, refill_quantum, tally_copies), makes it the context's
  * default, and returns its photons/s in *rate. */
 octmps_status octmps_autotune(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p,
