@@ -25,7 +25,7 @@ HBM_PEAK_GBS = 8000.0    # MI355X HBM3E nominal
 # (x2.0, the factor by which that counter under-reads a copy kernel of known size) plus
 # WRITE_SIZE (x1.0), divided by the launch's photon-steps. profiles/transport_slab4_pmc.md
 # holds the counter values and commands.
-TRAFFIC_B_PER_STEP = 32.14
+TRAFFIC_B_PER_STEP = 31.54
 
 
 def parse():
