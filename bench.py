@@ -21,6 +21,7 @@ sys.path.insert(0, os.path.join(ROOT, "oct-mps_amd"))
 
 ALG_BYTES_PER_STEP = 84  # one tet record per photon-step: 4 planes x 16 B + 4 links x 4 B + region 4 B
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E nominal
+LAYOUT_BYTES_PER_STEP = 80  # this build's packed tet record: 4 planes x 16 B + 4 packed neighbours x 4 B
 # Measured HBM bytes per photon-step of the batched slab4 launch: rocprofv3 --pmc FETCH_SIZE
 # (x2.0, the factor by which that counter under-reads a copy kernel of known size) plus
 # WRITE_SIZE (x1.0), divided by the launch's photon-steps. profiles/transport_slab4_pmc.md
@@ -281,7 +282,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (steps_per_launch * TRAFFIC_B_PER_STEP) if TRAFFIC_B_PER_STEP else None,
-                     "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step"},
+                     "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step",
+                     "layout_bytes_per_step": LAYOUT_BYTES_PER_STEP},
         "cpu_baseline": cpu,
         "serving": serving,
     }

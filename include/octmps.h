@@ -204,7 +204,9 @@ typedef struct octmps_ctx octmps_ctx;
 octmps_status octmps_ctx_create(int32_t device, octmps_ctx **out);
 void octmps_ctx_destroy(octmps_ctx *ctx);
 /* Upload (replaces RunGPUi's per-A-scan cudaMalloc+H2D of the mesh, src/main.cu:100-465):
- * the mesh stays resident until released. */
+ * the mesh stays resident until released. All resident meshes share one pool of at most
+ * 2^25 - 1 tets (neighbour ids are packed with the neighbour's region), and regions must
+ * be < 100 (OCTMPS_E_RANGE otherwise). */
 octmps_status octmps_ctx_upload_mesh(octmps_ctx *ctx, const octmps_mesh *m, int32_t *mesh_id);
 octmps_status octmps_ctx_release_mesh(octmps_ctx *ctx, int32_t mesh_id);
 /* Device bytes in use (meshes + work buffers) and HBM total. */

@@ -4,8 +4,10 @@
 // meshes so one launch can walk several meshes):
 //   planes[t*4 + j] : float4 (nx, ny, nz, d), face j pre-signed so n.x + d >= 0 inside
 //                     (64 B per tet, 64-B aligned)
-//   topo[t]         : links[4] neighbour tet or -1, nbr_regions (4 x u8, 0xFF at the mesh
-//                     boundary), region, 2 pad words (32 B per tet)
+//   topo[t]         : 4 packed neighbour words, one per face: bits 0-24 the neighbour's
+//                     pool tet id, bits 25-31 its region; 0xFFFFFFFF at the mesh
+//                     boundary (16 B per tet)
+// One tet visit is then 5 loads of 16 B (80 B): the 4 planes and the 4 packed neighbours.
 // Reference layout being replaced: TetrahedronStructGPU + TriangleFaces (src/octmps.h:52-57,
 // 193-196), read as 4 strided face gathers per step (src/octmps_kernel.cuh:517-545).
 #pragma once
@@ -17,14 +19,14 @@
 namespace octmps {
 
 constexpr int kMaxRegions = 100;
+constexpr uint32_t kLinkBits = 25;                       // pool tet id bits of a packed link
+constexpr uint32_t kLinkMask = (1u << kLinkBits) - 1u;   // also the boundary id
+constexpr int64_t kMaxPoolTets = (int64_t)kLinkMask;     // ids 0 .. 2^25 - 2
 
 struct TopoRec {
-    int32_t link[4];
-    uint32_t nbr_regions;
-    int32_t region;
-    int32_t pad[2];
+    uint32_t link[4];  // (region << 25) | pool tet id, or 0xFFFFFFFF at the mesh boundary
 };
-static_assert(sizeof(TopoRec) == 32, "TopoRec must be 32 bytes");
+static_assert(sizeof(TopoRec) == 16, "TopoRec must be 16 bytes");
 
 // Per-region constants of one parameter set (RegionStructGPU, src/octmps_kernel.h:132-140).
 struct DevRegion {

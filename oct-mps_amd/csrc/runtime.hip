@@ -146,7 +146,7 @@ int default_block(octmps_ctx *ctx) { return ctx->tuned_block > 0 ? ctx->tuned_bl
 // Untuned defaults: measured best on the four-layer slab B-scan (128 x 3e5 photons), where
 // fuller refills and scatter phases beat the lane utilisation they give up.
 int default_quantum(octmps_ctx *ctx) { return ctx->tuned_q > 0 ? ctx->tuned_q : 16; }
-int default_scatter(octmps_ctx *ctx) { return ctx->tuned_sq > 0 ? ctx->tuned_sq : 44; }
+int default_scatter(octmps_ctx *ctx) { return ctx->tuned_sq > 0 ? ctx->tuned_sq : 40; }
 int default_copies(octmps_ctx *ctx) { return ctx->tuned_copies > 0 ? ctx->tuned_copies : 1; }
 
 void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid) {
@@ -373,10 +373,11 @@ extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh
     if (!c || !mh || !mesh_id) return fail(OCTMPS_E_INVALID, "null argument");
     const HostMesh &m = mh->m;
     const int64_t T = m.ntets();
-    if (m.max_region >= 255) return fail(OCTMPS_E_RANGE, "regions must be < 255");
+    if (m.max_region >= kMaxRegions) return fail(OCTMPS_E_RANGE, "regions must be < 100");
     std::unique_lock<std::shared_mutex> pk(c->pool_mu);
     hipSetDevice(c->device);
-    if (c->pool_used + T > (int64_t)INT32_MAX) return fail(OCTMPS_E_RANGE, "mesh pool exceeds 2^31 tets");
+    if (c->pool_used + T > kMaxPoolTets)
+        return fail(OCTMPS_E_RANGE, "resident meshes exceed 2^25 - 1 tets (packed neighbour ids)");
     hipError_t e;
     if (c->pool_used + T > c->pool_cap) {
         int64_t ncap = std::max<int64_t>(c->pool_used + T, c->pool_cap * 2);
@@ -411,16 +412,11 @@ extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh
             planes[4 * t + j] = make_float4((float)p[0], (float)p[1], (float)p[2], (float)p[3]);
         }
         TopoRec &r = topo[t];
-        uint32_t nr = 0;
         for (int j = 0; j < 4; ++j) {
-            int32_t l = m.links[4 * t + j];
-            r.link[j] = l < 0 ? -1 : (int32_t)(l + off);
-            uint32_t reg = l < 0 ? 0xffu : (uint32_t)m.region[l];
-            nr |= reg << (8 * j);
+            const int32_t l = m.links[4 * t + j];
+            r.link[j] = l < 0 ? 0xffffffffu
+                              : ((uint32_t)m.region[l] << kLinkBits) | (uint32_t)(l + off);
         }
-        r.nbr_regions = nr;
-        r.region = m.region[t];
-        r.pad[0] = r.pad[1] = 0;
     }
     if ((e = hipMemcpy(c->pool_planes + off * 4, planes.data(), planes.size() * sizeof(float4),
                        hipMemcpyHostToDevice)) != hipSuccess ||
@@ -503,7 +499,7 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     octmps_batch_opts win = *opts;
     win.block_size = 256;
     win.refill_quantum = 16;
-    win.scatter_quantum = 44;
+    win.scatter_quantum = 40;
     win.tally_copies = 1;
     win.reserved = 0;
     win.use_graph = 0;
@@ -519,7 +515,7 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
         return OCTMPS_OK;
     };
     octmps_status s;
-    for (int sq : {32, 40, 44, 48}) {
+    for (int sq : {32, 36, 40, 44}) {
         octmps_batch_opts o = win;
         o.scatter_quantum = sq;
         if ((s = trial(o)) != OCTMPS_OK) return s;

@@ -387,8 +387,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             // ---- HitBoundary, src/octmps_kernel.cuh:517-572
             const float4 *P = A.planes + (size_t)tet * 4;
             const float4 p0 = P[0], p1 = P[1], p2 = P[2], p3 = P[3];
-            const int4 L = *reinterpret_cast<const int4 *>(&A.topo[tet].link[0]);
-            const uint32_t nregs = A.topo[tet].nbr_regions;
+            const uint4 L = *reinterpret_cast<const uint4 *>(&A.topo[tet].link[0]);
             const float c0 = fmaf(p0.z, uz, fmaf(p0.y, uy, p0.x * ux));
             const float c1 = fmaf(p1.z, uz, fmaf(p1.y, uy, p1.x * ux));
             const float c2 = fmaf(p2.z, uz, fmaf(p2.y, uy, p2.x * ux));
@@ -426,8 +425,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             if (hit) {
                 // ---- FastReflectTransmit, src/octmps_kernel.cuh:590-805
                 const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
-                const int nxt = imin == 0 ? L.x : (imin == 1 ? L.y : (imin == 2 ? L.z : L.w));
-                const int nr = (int)((nregs >> (8 * imin)) & 0xffu);
+                // Packed neighbour: pool tet id (-1 at the mesh boundary) and its region.
+                const uint32_t pk = imin == 0 ? L.x : (imin == 1 ? L.y : (imin == 2 ? L.z : L.w));
+                const int nxt = (pk & kLinkMask) == kLinkMask ? -1 : (int)(pk & kLinkMask);
+                const int nr = (int)(pk >> kLinkBits);
                 const float ca1 = -mincos;
                 // The empty asm keeps rn a register value: otherwise the three candidate
                 // loads are merged into one load through a phi of pointers, which puts rn
