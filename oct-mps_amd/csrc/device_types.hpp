@@ -1,13 +1,15 @@
 // Layouts shared by the HIP transport kernel (transport.hip) and the host runtime.
 //
-// Mesh in HBM (structure of arrays, one record per tet, global tet ids across all resident
-// meshes so one launch can walk several meshes):
-//   planes[t*4 + j] : float4 (nx, ny, nz, d), face j pre-signed so n.x + d >= 0 inside
-//                     (64 B per tet, 64-B aligned)
-//   topo[t]         : 4 packed neighbour words, one per face: bits 0-24 the neighbour's
-//                     pool tet id, bits 25-31 its region; 0xFFFFFFFF at the mesh
-//                     boundary (16 B per tet)
-// One tet visit is then 5 loads of 16 B (80 B): the 4 planes and the 4 packed neighbours.
+// Mesh in HBM: one 128-byte record per tet (TetRec), global tet ids across all resident
+// meshes so one launch can walk several meshes:
+//   plane[j] : float4 (nx, ny, nz, d), face j pre-signed so n.x + d >= 0 inside (64 B)
+//   link[j]  : packed neighbour word of face j: bits 0-24 the neighbour's pool tet id,
+//              bits 25-31 its region; 0xFFFFFFFF at the mesh boundary (16 B)
+//   pad      : 48 B, so that a record is exactly one 128-byte cache line
+// One tet visit is then 5 loads of 16 B (80 B) that all fall in one cache line. With the
+// planes and the neighbour words in two arrays, a visit touched two lines, and the L1
+// cache (which holds far less than the records of the photons in flight on a CU) missed
+// on both.
 // Reference layout being replaced: TetrahedronStructGPU + TriangleFaces (src/octmps.h:52-57,
 // 193-196), read as 4 strided face gathers per step (src/octmps_kernel.cuh:517-545).
 #pragma once
@@ -23,10 +25,12 @@ constexpr uint32_t kLinkBits = 25;                       // pool tet id bits of 
 constexpr uint32_t kLinkMask = (1u << kLinkBits) - 1u;   // also the boundary id
 constexpr int64_t kMaxPoolTets = (int64_t)kLinkMask;     // ids 0 .. 2^25 - 2
 
-struct TopoRec {
+struct __align__(128) TetRec {
+    float4 plane[4];
     uint32_t link[4];  // (region << 25) | pool tet id, or 0xFFFFFFFF at the mesh boundary
+    uint32_t pad[12];
 };
-static_assert(sizeof(TopoRec) == 16, "TopoRec must be 16 bytes");
+static_assert(sizeof(TetRec) == 128, "TetRec must be one 128-byte line");
 
 // Per-region constants of one parameter set (RegionStructGPU, src/octmps_kernel.h:132-140).
 struct DevRegion {
@@ -119,8 +123,7 @@ struct PhotonLogRec {
 enum { CTRL_NEXT_ID = 0, CTRL_STEPS = 1, CTRL_KILLED = 2, CTRL_DONE = 3, CTRL_ITERS = 4, CTRL_WORDS = 8 };
 
 struct KArgs {
-    const float4 *planes;
-    const TopoRec *topo;
+    const TetRec *tets;
     const DevRegion *regions;
     const DevSet *sets;
     const DevScan *slots;

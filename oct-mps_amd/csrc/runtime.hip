@@ -229,8 +229,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
         return hip_fail(e, "upload");
     KArgs A;
     std::memset(&A, 0, sizeof A);
-    A.planes = ctx->pool_planes;
-    A.topo = ctx->pool_topo;
+    A.tets = ctx->pool_tets;
     A.regions = (const DevRegion *)W.regions.p;
     A.sets = (const DevSet *)W.sets.p;
     A.slots = (const DevScan *)W.slots.p;
@@ -360,8 +359,7 @@ extern "C" void octmps_ctx_destroy(octmps_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->graph.exec) hipGraphExecDestroy(c->graph.exec);
     c->work.release_all();
-    if (c->pool_planes) hipFree(c->pool_planes);
-    if (c->pool_topo) hipFree(c->pool_topo);
+    if (c->pool_tets) hipFree(c->pool_tets);
     if (c->d_mult) hipFree(c->d_mult);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -381,22 +379,13 @@ extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh
     hipError_t e;
     if (c->pool_used + T > c->pool_cap) {
         int64_t ncap = std::max<int64_t>(c->pool_used + T, c->pool_cap * 2);
-        float4 *np = nullptr;
-        TopoRec *nt = nullptr;
-        if ((e = hipMalloc(&np, (size_t)ncap * 4 * sizeof(float4))) != hipSuccess)
+        TetRec *np = nullptr;
+        if ((e = hipMalloc(&np, (size_t)ncap * sizeof(TetRec))) != hipSuccess)
             return fail(OCTMPS_E_NOMEM, "mesh pool allocation failed");
-        if ((e = hipMalloc(&nt, (size_t)ncap * sizeof(TopoRec))) != hipSuccess) {
-            hipFree(np);
-            return fail(OCTMPS_E_NOMEM, "mesh pool allocation failed");
-        }
-        if (c->pool_used > 0) {
-            hipMemcpy(np, c->pool_planes, (size_t)c->pool_used * 4 * sizeof(float4), hipMemcpyDeviceToDevice);
-            hipMemcpy(nt, c->pool_topo, (size_t)c->pool_used * sizeof(TopoRec), hipMemcpyDeviceToDevice);
-        }
-        if (c->pool_planes) hipFree(c->pool_planes);
-        if (c->pool_topo) hipFree(c->pool_topo);
-        c->pool_planes = np;
-        c->pool_topo = nt;
+        if (c->pool_used > 0)
+            hipMemcpy(np, c->pool_tets, (size_t)c->pool_used * sizeof(TetRec), hipMemcpyDeviceToDevice);
+        if (c->pool_tets) hipFree(c->pool_tets);
+        c->pool_tets = np;
         c->pool_cap = ncap;
         if (c->graph.exec) {
             hipGraphExecDestroy(c->graph.exec);
@@ -404,24 +393,20 @@ extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh
         }
     }
     const int64_t off = c->pool_used;
-    std::vector<float4> planes((size_t)T * 4);
-    std::vector<TopoRec> topo((size_t)T);
+    std::vector<TetRec> recs((size_t)T);
     for (int64_t t = 0; t < T; ++t) {
+        TetRec &r = recs[t];
+        std::memset(&r, 0, sizeof r);
         for (int j = 0; j < 4; ++j) {
             const double *p = &m.planes[16 * t + 4 * j];
-            planes[4 * t + j] = make_float4((float)p[0], (float)p[1], (float)p[2], (float)p[3]);
-        }
-        TopoRec &r = topo[t];
-        for (int j = 0; j < 4; ++j) {
+            r.plane[j] = make_float4((float)p[0], (float)p[1], (float)p[2], (float)p[3]);
             const int32_t l = m.links[4 * t + j];
             r.link[j] = l < 0 ? 0xffffffffu
                               : ((uint32_t)m.region[l] << kLinkBits) | (uint32_t)(l + off);
         }
     }
-    if ((e = hipMemcpy(c->pool_planes + off * 4, planes.data(), planes.size() * sizeof(float4),
-                       hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(c->pool_topo + off, topo.data(), topo.size() * sizeof(TopoRec), hipMemcpyHostToDevice)) !=
-            hipSuccess)
+    if ((e = hipMemcpy(c->pool_tets + off, recs.data(), recs.size() * sizeof(TetRec), hipMemcpyHostToDevice)) !=
+        hipSuccess)
         return hip_fail(e, "mesh upload");
     MeshEntry me;
     me.offset = off;
@@ -453,7 +438,7 @@ extern "C" octmps_status octmps_ctx_release_mesh(octmps_ctx *c, int32_t id) {
 
 extern "C" octmps_status octmps_ctx_memory(octmps_ctx *c, uint64_t *used, uint64_t *total) {
     if (!c) return fail(OCTMPS_E_INVALID, "null argument");
-    uint64_t u = (uint64_t)c->pool_cap * (4 * sizeof(float4) + sizeof(TopoRec)) + c->work.bytes();
+    uint64_t u = (uint64_t)c->pool_cap * sizeof(TetRec) + c->work.bytes();
     if (used) *used = u;
     if (total) *total = c->hbm_total;
     return OCTMPS_OK;

@@ -66,8 +66,7 @@ struct octmps_ctx {
     uint32_t *d_mult = nullptr;
     // Resident mesh pool (global tet ids).
     std::shared_mutex pool_mu;  // shared: launches and submits; exclusive: mesh upload/release
-    float4 *pool_planes = nullptr;
-    octmps::TopoRec *pool_topo = nullptr;
+    octmps::TetRec *pool_tets = nullptr;
     int64_t pool_cap = 0, pool_used = 0;
     std::vector<octmps::MeshEntry> meshes;
     // Plain/batched runs.

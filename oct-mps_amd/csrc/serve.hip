@@ -275,8 +275,7 @@ void scheduler(octmps_server *s) {
                   (e = launch_fill_u64((uint64_t *)s->ctrl.p, CTRL_WORDS, 0, s->stream)) == hipSuccess;
         KArgs A;
         std::memset(&A, 0, sizeof A);
-        A.planes = ctx->pool_planes;
-        A.topo = ctx->pool_topo;
+        A.tets = ctx->pool_tets;
         A.regions = (const DevRegion *)s->regions.p;
         A.sets = (const DevSet *)s->sets.p;
         A.slots = (const DevScan *)s->slots.p;

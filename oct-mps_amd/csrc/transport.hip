@@ -32,8 +32,10 @@ constexpr float kChanceF = 0.1f;        // CHANCE, src/octmps.h:23
 constexpr float kNormalSin2 = 1e-12f;
 
 // Lane flags: photon alive, first backward bias done, a backward specular reflection seen,
-// the continuation photon's copy of that bit, scatter pending.
-constexpr uint32_t F_ALIVE = 1u, F_FST = 2u, F_NBS = 4u, F_CNBS = 8u, F_PEND = 16u;
+// the continuation photon's copy of that bit, scatter pending, interface pending (the exit
+// face index 0-3 in bits kFaceShift..kFaceShift+1).
+constexpr uint32_t F_ALIVE = 1u, F_FST = 2u, F_NBS = 4u, F_CNBS = 8u, F_PEND = 16u, F_IFACE = 32u;
+constexpr uint32_t kFaceShift = 6, F_FACE = 3u << kFaceShift;
 
 // __uint2float_rz(u) / 2^32: keep the top 24 significant bits, then an exact scale.
 __device__ __forceinline__ float unit_rz24(uint32_t u) {
@@ -43,11 +45,14 @@ __device__ __forceinline__ float unit_rz24(uint32_t u) {
     return (float)t * 2.3283064365386963e-10f;
 }
 
-// rand_MWC_co / rand_MWC_oc, src/octmps_kernel.cuh:10-29.
-__device__ __forceinline__ float rand_co(uint64_t &x, uint32_t a) {
+// One multiply-with-carry step; returns the low word that the draw converts.
+__device__ __forceinline__ uint32_t mwc_next(uint64_t &x, uint32_t a) {
     x = (x & 0xffffffffull) * (uint64_t)a + (x >> 32);
-    return unit_rz24((uint32_t)x);
+    return (uint32_t)x;
 }
+
+// rand_MWC_co / rand_MWC_oc, src/octmps_kernel.cuh:10-29.
+__device__ __forceinline__ float rand_co(uint64_t &x, uint32_t a) { return unit_rz24(mwc_next(x, a)); }
 __device__ __forceinline__ float rand_oc(uint64_t &x, uint32_t a) { return 1.0f - rand_co(x, a); }
 
 // Hardware reciprocal (v_rcp_f32, 1 ulp); the reference divides with round-down FP64
@@ -227,6 +232,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
         for (int i = threadIdx.x; i < kMaxRegions; i += blockDim.x) lds_regions[i] = A.regions[S0.region_base + i];
         __syncthreads();
     }
+    // Per-lane LDS stash (blocks have at most 256 lanes): the probe position of the lane's
+    // A-scan, written at launch, and the exit face of a lane parked at an interface (normal
+    // and packed neighbour word), written when it parks, so that the parked phase reads them
+    // from LDS instead of making a dependent global round trip.
+    __shared__ float4 lds_probe[256], lds_face[256];
     auto region_rec = [&](int r) -> const DevRegion & {
         // The region table of set s starts at s * kMaxRegions (DevSet::region_base).
         return MULTI ? A.regions[set * kMaxRegions + r] : lds_regions[r];
@@ -251,6 +261,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             slot = s.slot; id = s.id; ndet = s.ndet; csum = s.csum;
             set = MULTI ? A.slots[slot].set : 0;
             load_region(region);
+            const DevScan &sc = A.slots[slot];
+            lds_probe[threadIdx.x] = make_float4(sc.px, sc.py, sc.pz, 0.0f);
+            if (flags & F_IFACE) {
+                const uint32_t f = (flags & F_FACE) >> kFaceShift;
+                const float4 pl = A.tets[tet].plane[f];
+                lds_face[threadIdx.x] = make_float4(pl.x, pl.y, pl.z, __uint_as_float(A.tets[tet].link[f]));
+            }
         }
     }
 
@@ -297,9 +314,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                         sR = derive_state(seeding_state(h, 1, b1), b1, aR);
                         sS = derive_state(seeding_state(h, 2, b2), b2, aS);
                         x = sc.px; y = sc.py; z = sc.pz;
+                        lds_probe[threadIdx.x] = make_float4(sc.px, sc.py, sc.pz, 0.0f);
                         ux = 0.0f; uy = 0.0f; uz = 1.0f;
                         w = sc.w0;
-                        tau = 0.0f; op = 0.0f; maxz = 0.0f;
+                        // The first step length (General stream) is drawn at launch; the
+                        // reference draws it at the top of the first step, with no other
+                        // General draw in between.
+                        tau = neg_ln(rand_oc(sG, aG));
+                        op = 0.0f; maxz = 0.0f;
                         lr = 1.0f; lr_fst = 1.0f; loc_fst = -1.0f;
                         tet = sc.root_tet;
                         region = sc.root_region;
@@ -329,11 +351,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             __builtin_amdgcn_s_memrealtime() - t_start > A.slice_ticks)
             break;
         ++iters;
-        // A lane whose step ended inside the tet (absorb + scatter) parks with F_PEND until
-        // enough lanes of the wave need the scattering code (scatter quantum), while the
-        // other lanes keep crossing faces. Each photon still runs its operations in the
-        // reference's order, so results do not depend on the quantum.
-        const bool ready = (flags & (F_ALIVE | F_PEND)) == F_ALIVE;
+        // The crossing phase handles the common step: a hop that ends inside the tet or on
+        // a face into the same region. A lane whose step ended inside the tet (absorb +
+        // scatter) parks with F_PEND, and one that reached a face into another region, the
+        // mesh boundary, or a face at grazing incidence parks with F_IFACE (Fresnel,
+        // reflection, detection), until enough lanes of the wave need the parked phase
+        // (scatter quantum), while the other lanes keep crossing faces. Each photon still
+        // runs its operations in the reference's order, so results do not depend on the
+        // quantum.
+        const bool ready = (flags & (F_ALIVE | F_PEND | F_IFACE)) == F_ALIVE;
         wave_steps += (uint64_t)__popcll(__ballot(ready));
 
         const DevSet &S = MULTI ? A.sets[set] : S0;
@@ -379,15 +405,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             finish = true;
             killed = true;
         } else {
-            // ---- ComputeStepSize, src/octmps_kernel.cuh:505-515
+            // ---- ComputeStepSize, src/octmps_kernel.cuh:505-515. A fresh step length
+            // (sleft == 0) is normally drawn ahead, where the previous step ended (parked
+            // phase, launch), so this phase rarely runs the logarithm. +0 marks "draw now";
+            // a drawn length of zero is -0 (-ln(1)) and is used as is.
             float tau_d = tau;
-            if (tau_d == 0.0f) tau_d = neg_ln(rand_oc(sG, aG));
+            if (__float_as_uint(tau_d) == 0u) tau_d = neg_ln(rand_oc(sG, aG));
             float s = tau_d * rmuas;
             tau = 0.0f;
             // ---- HitBoundary, src/octmps_kernel.cuh:517-572
-            const float4 *P = A.planes + (size_t)tet * 4;
-            const float4 p0 = P[0], p1 = P[1], p2 = P[2], p3 = P[3];
-            const uint4 L = *reinterpret_cast<const uint4 *>(&A.topo[tet].link[0]);
+            const TetRec &TR = A.tets[tet];
+            const float4 p0 = TR.plane[0], p1 = TR.plane[1], p2 = TR.plane[2], p3 = TR.plane[3];
+            const uint4 L = *reinterpret_cast<const uint4 *>(&TR.link[0]);
             const float c0 = fmaf(p0.z, uz, fmaf(p0.y, uy, p0.x * ux));
             const float c1 = fmaf(p1.z, uz, fmaf(p1.y, uy, p1.x * ux));
             const float c2 = fmaf(p2.z, uz, fmaf(p2.y, uy, p2.x * ux));
@@ -411,8 +440,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             const bool hit = imin >= 0 && s > dmin;
             if (hit) {
                 // Glass (mus = 0) keeps the leftover unchanged, as (s - d) / DBL_MAX does
-                // in the reference; an overflowed step stays infinite.
-                tau = (muas == 0.0f) ? (isinf(s) ? INFINITY : tau_d) : (s - dmin) * muas;
+                // in the reference; an overflowed step stays infinite, and a zero leftover
+                // is stored as +0 so that the next step draws.
+                tau = (muas == 0.0f) ? (isinf(s) ? INFINITY : tau_d + 0.0f) : (s - dmin) * muas;
                 s = dmin;
             }
             // ---- Hop, src/octmps_kernel.cuh:574-582
@@ -423,13 +453,53 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             if (maxz < z) maxz = z;
 
             if (hit) {
-                // ---- FastReflectTransmit, src/octmps_kernel.cuh:590-805
-                const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
-                // Packed neighbour: pool tet id (-1 at the mesh boundary) and its region.
+                // Packed neighbour: pool tet id (all ones at the mesh boundary) and region.
                 const uint32_t pk = imin == 0 ? L.x : (imin == 1 ? L.y : (imin == 2 ? L.z : L.w));
+                if ((pk & kLinkMask) != kLinkMask && (int)(pk >> kLinkBits) == region && -mincos >= kCos90F) {
+                    // FastReflectTransmit (src/octmps_kernel.cuh:590-805) on a face into the
+                    // same region away from grazing incidence: equal indices give R = 0
+                    // (sam == 0), so the photon transmits unless the ReflectT draw is 0,
+                    // which it is exactly when the new MWC word is 0.
+                    if (mwc_next(sR, aR) != 0u) {
+                        tet = (int)(pk & kLinkMask);
+                    } else {
+                        const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
+                        const float two_ca1 = -2.0f * mincos;
+                        if (uz > 0.0f) flags |= F_NBS;
+                        ux = fmaf(two_ca1, pf.x, ux);
+                        uy = fmaf(two_ca1, pf.y, uy);
+                        uz = fmaf(two_ca1, pf.z, uz);
+                    }
+                } else {
+                    const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
+                    lds_face[threadIdx.x] = make_float4(pf.x, pf.y, pf.z, __uint_as_float(pk));
+                    flags |= F_IFACE | ((uint32_t)imin << kFaceShift);
+                }
+            } else {
+                flags |= F_PEND;
+            }
+        }
+        // Roulette after a step completed in the crossing phase.
+        if (ready && !finish && !(flags & (F_PEND | F_IFACE))) finish = roulette();
+
+        // ---- Parked phase: interface events (Fresnel, reflection or transmission,
+        // detection), or Drop + Spin/SpinBias, then roulette, for the parked lanes.
+        const uint64_t pmask = __ballot((flags & (F_PEND | F_IFACE)) != 0);
+        if (pmask != 0 && ((uint32_t)__popcll(pmask) >= A.scatter_q ||
+                           __ballot((flags & (F_ALIVE | F_PEND | F_IFACE)) == F_ALIVE) == 0)) {
+            const bool parked = (flags & (F_PEND | F_IFACE)) != 0;
+            if (flags & F_IFACE) {
+                // ---- FastReflectTransmit, src/octmps_kernel.cuh:590-805
+                flags &= ~(F_IFACE | F_FACE);
+                // The exit face's normal and packed neighbour word (pool tet id, all ones at
+                // the mesh boundary, and its region), stashed when the lane parked.
+                const float4 fr = lds_face[threadIdx.x];
+                const float3 pf = make_float3(fr.x, fr.y, fr.z);
+                const uint32_t pk = __float_as_uint(fr.w);
                 const int nxt = (pk & kLinkMask) == kLinkMask ? -1 : (int)(pk & kLinkMask);
                 const int nr = (int)(pk >> kLinkBits);
-                const float ca1 = -mincos;
+                // The same operations as the crossing phase's cosine of this face.
+                const float ca1 = -fmaf(pf.z, uz, fmaf(pf.y, uy, pf.x * ux));
                 // The empty asm keeps rn a register value: otherwise the three candidate
                 // loads are merged into one load through a phi of pointers, which puts rn
                 // in scratch memory and turns the load into a flat load.
@@ -448,8 +518,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                 if (nxt < 0 && R < 1.0f) {
                     if (uz < 0.0f) {
                         bool flag = false;
-                        const DevScan &SC = A.slots[slot];
-                        const float dx = x - SC.px, dy = y - SC.py;
+                        const float4 probe = lds_probe[threadIdx.x];
+                        const float dx = x - probe.x, dy = y - probe.y;
                         if (fmaf(dx, dx, dy * dy) < S.radius2 && -uz > S.cos_angle) {
                             const int tb = S.type_bias;
                             if (tb == 0 || tb == 3) flag = true;
@@ -485,16 +555,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                     uz = rz;
                 }
                 finish = roulette();
-            } else {
-                flags |= F_PEND;
-            }
-        }
-
-        // ---- Scatter phase: Drop + Spin/SpinBias + roulette for the parked lanes.
-        const uint64_t pmask = __ballot((flags & F_PEND) != 0);
-        if (pmask != 0 &&
-            ((uint32_t)__popcll(pmask) >= A.scatter_q || __ballot((flags & (F_ALIVE | F_PEND)) == F_ALIVE) == 0)) {
-            if (flags & F_PEND) {
+            } else if (flags & F_PEND) {
                 flags &= ~F_PEND;
                 // ---- Drop, src/octmps_kernel.cuh:584-588
                 const DevRegion RG = region_rec(region);
@@ -521,8 +582,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                         atmp = S.bias_a;
                         first = true;
                     }
-                    const DevScan &SC = A.slots[slot];
-                    float vx = SC.px - x, vy = SC.py - y, vz = SC.pz - z;
+                    const float4 probe = lds_probe[threadIdx.x];
+                    float vx = probe.x - x, vy = probe.y - y, vz = probe.z - z;
                     const float len2 = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
                     const float rl = rsqrtf(len2);
                     vx *= rl;
@@ -578,6 +639,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                 }
                 finish = roulette();
             }
+            // ComputeStepSize's draw for the next step of a scattered or restored photon,
+            // made here: its next General draw is that one.
+            if (parked && !finish && __float_as_uint(tau) == 0u) tau = neg_ln(rand_oc(sG, aG));
         }
         if (finish) {
             flags = 0;
