@@ -3,9 +3,9 @@
 // Mesh in HBM: one 128-byte record per tet (TetRec), global tet ids across all resident
 // meshes so one launch can walk several meshes:
 //   plane[j] : float4 (nx, ny, nz, d), face j pre-signed so n.x + d >= 0 inside (64 B)
-//   link[j]  : packed neighbour word of face j: bits 0-24 the neighbour's pool tet id,
-//              bits 25-31 its region; 0xFFFFFFFF at the mesh boundary (16 B)
-//   pad      : 48 B, so that a record is exactly one 128-byte cache line
+//   link[j]  : packed neighbour word of face j (below; 16 B)
+//   region   : the tet's own region, read only through escaped links (4 B)
+//   pad      : 44 B, so that a record is exactly one 128-byte cache line
 // One tet visit is then 5 loads of 16 B (80 B) that all fall in one cache line. With the
 // planes and the neighbour words in two arrays, a visit touched two lines, and the L1
 // cache (which holds far less than the records of the photons in flight on a CU) missed
@@ -24,11 +24,21 @@ constexpr int kMaxRegions = 100;
 constexpr uint32_t kLinkBits = 25;                       // pool tet id bits of a packed link
 constexpr uint32_t kLinkMask = (1u << kLinkBits) - 1u;   // also the boundary id
 constexpr int64_t kMaxPoolTets = (int64_t)kLinkMask;     // ids 0 .. 2^25 - 2
+// Packed neighbour word: bits 0-24 the neighbour's pool tet id, bits 25-26 the index of the
+// shared face within the neighbour, bits 27-31 the neighbour's region, or kRegionEscape
+// when that region is >= 31, when the neighbour does not link back through that face, or
+// when the two copies of the shared plane are not exact negations (the kernel then reads
+// the region from the neighbour's record and does not use the face index). 0xFFFFFFFF at
+// the mesh boundary.
+constexpr uint32_t kBackShift = 25;
+constexpr uint32_t kRegionShift = 27;
+constexpr uint32_t kRegionEscape = 31;
 
 struct __align__(128) TetRec {
     float4 plane[4];
-    uint32_t link[4];  // (region << 25) | pool tet id, or 0xFFFFFFFF at the mesh boundary
-    uint32_t pad[12];
+    uint32_t link[4];
+    int32_t region;
+    uint32_t pad[11];
 };
 static_assert(sizeof(TetRec) == 128, "TetRec must be one 128-byte line");
 
@@ -124,6 +134,7 @@ enum { CTRL_NEXT_ID = 0, CTRL_STEPS = 1, CTRL_KILLED = 2, CTRL_DONE = 3, CTRL_IT
 
 struct KArgs {
     const TetRec *tets;
+    uint32_t tets_bytes;        // bytes of resident records (<= (2^25 - 1) * 128 < 2^32)
     const DevRegion *regions;
     const DevSet *sets;
     const DevScan *slots;

@@ -230,6 +230,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     KArgs A;
     std::memset(&A, 0, sizeof A);
     A.tets = ctx->pool_tets;
+    A.tets_bytes = (uint32_t)(ctx->pool_used * (int64_t)sizeof(TetRec));
     A.regions = (const DevRegion *)W.regions.p;
     A.sets = (const DevSet *)W.sets.p;
     A.slots = (const DevScan *)W.slots.p;
@@ -400,9 +401,29 @@ extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh
         for (int j = 0; j < 4; ++j) {
             const double *p = &m.planes[16 * t + 4 * j];
             r.plane[j] = make_float4((float)p[0], (float)p[1], (float)p[2], (float)p[3]);
+        }
+        r.region = m.region[t];
+    }
+    for (int64_t t = 0; t < T; ++t) {
+        TetRec &r = recs[t];
+        for (int j = 0; j < 4; ++j) {
             const int32_t l = m.links[4 * t + j];
-            r.link[j] = l < 0 ? 0xffffffffu
-                              : ((uint32_t)m.region[l] << kLinkBits) | (uint32_t)(l + off);
+            if (l < 0) {
+                r.link[j] = 0xffffffffu;
+                continue;
+            }
+            // The face of l that links back to t through the exactly negated plane: a
+            // photon that crosses into l through it moves away from it (its cosine there is
+            // the negation of a negative one), so its next exit is one of the other three.
+            uint32_t back = 4;
+            for (int k = 0; k < 4 && back == 4; ++k) {
+                const float4 a = r.plane[j], b = recs[l].plane[k];
+                if (m.links[4 * (int64_t)l + k] == t && a.x == -b.x && a.y == -b.y && a.z == -b.z && a.w == -b.w)
+                    back = (uint32_t)k;
+            }
+            const uint32_t reg =
+                (back < 4 && m.region[l] < (int32_t)kRegionEscape) ? (uint32_t)m.region[l] : kRegionEscape;
+            r.link[j] = (reg << kRegionShift) | ((back & 3u) << kBackShift) | (uint32_t)(l + off);
         }
     }
     if ((e = hipMemcpy(c->pool_tets + off, recs.data(), recs.size() * sizeof(TetRec), hipMemcpyHostToDevice)) !=

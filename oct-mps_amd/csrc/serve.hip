@@ -276,6 +276,7 @@ void scheduler(octmps_server *s) {
         KArgs A;
         std::memset(&A, 0, sizeof A);
         A.tets = ctx->pool_tets;
+        A.tets_bytes = (uint32_t)(ctx->pool_used * (int64_t)sizeof(TetRec));
         A.regions = (const DevRegion *)s->regions.p;
         A.sets = (const DevSet *)s->sets.p;
         A.slots = (const DevScan *)s->slots.p;

@@ -36,6 +36,15 @@ constexpr float kNormalSin2 = 1e-12f;
 // face index 0-3 in bits kFaceShift..kFaceShift+1).
 constexpr uint32_t F_ALIVE = 1u, F_FST = 2u, F_NBS = 4u, F_CNBS = 8u, F_PEND = 16u, F_IFACE = 32u;
 constexpr uint32_t kFaceShift = 6, F_FACE = 3u << kFaceShift;
+// Entry face of the current tet (0: none, k + 1: the photon crossed into it through face k).
+constexpr uint32_t kEntryShift = 8, F_ENTRY = 7u << kEntryShift;
+
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+// Buffer-resource word 3 for raw (untyped) loads on CDNA: 32-bit data format.
+constexpr int kBufferWord3 = 0x00020000;
+// A record offset past every resident record (tets_bytes <= 2^32 - 128): a raw buffer load
+// there is out of range, so it makes no memory request and returns zeros.
+constexpr uint32_t kOutOfRange = 0xFFFFFF80u;
 
 // __uint2float_rz(u) / 2^32: keep the top 24 significant bits, then an exact scale.
 __device__ __forceinline__ float unit_rz24(uint32_t u) {
@@ -277,6 +286,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     // Step guard as a 32-bit count (the per-photon counter is 32-bit).
     const uint32_t max_steps = A.max_steps > 0xffffffffull ? 0xffffffffu : (uint32_t)A.max_steps;
     const uint32_t q = A.refill_q < 1u ? 1u : A.refill_q;
+    const __amdgpu_buffer_rsrc_t tets_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)A.tets, (short)0, (int)A.tets_bytes, kBufferWord3);
 
     for (;;) {
         // ---- refill dead lanes (wave-uniform decision)
@@ -414,9 +425,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             float s = tau_d * rmuas;
             tau = 0.0f;
             // ---- HitBoundary, src/octmps_kernel.cuh:517-572
-            const TetRec &TR = A.tets[tet];
-            const float4 p0 = TR.plane[0], p1 = TR.plane[1], p2 = TR.plane[2], p3 = TR.plane[3];
-            const uint4 L = *reinterpret_cast<const uint4 *>(&TR.link[0]);
+            // The tet record is read through a buffer resource (32-bit offsets). A photon that
+            // crossed into this tet through face k does not fetch plane k: that plane is the
+            // exact negation of the one it crossed, so its cosine is > 0 and the face is never
+            // a candidate. Its load is sent out of range instead, which costs no memory
+            // request and returns a zero plane (cosine 0, also never a candidate).
+            const uint32_t entry = (flags & F_ENTRY) >> kEntryShift;
+            flags &= ~F_ENTRY;
+            const uint32_t rec = (uint32_t)tet << 7;
+            auto plane_at = [&](uint32_t k) -> float4 {
+                const u4v v = __builtin_amdgcn_raw_buffer_load_b128(
+                    tets_rsrc, entry == k + 1u ? kOutOfRange : rec, (int)(16u * k), 0);
+                return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                                   __uint_as_float(v.w));
+            };
+            const float4 p0 = plane_at(0), p1 = plane_at(1), p2 = plane_at(2), p3 = plane_at(3);
+            const u4v lv = __builtin_amdgcn_raw_buffer_load_b128(tets_rsrc, rec + 64u, 0, 0);
+            const uint4 L = make_uint4(lv.x, lv.y, lv.z, lv.w);
+            // Keep the five loads together ahead of the face tests (one memory round trip).
+            __builtin_amdgcn_sched_barrier(0);
             const float c0 = fmaf(p0.z, uz, fmaf(p0.y, uy, p0.x * ux));
             const float c1 = fmaf(p1.z, uz, fmaf(p1.y, uy, p1.x * ux));
             const float c2 = fmaf(p2.z, uz, fmaf(p2.y, uy, p2.x * ux));
@@ -455,13 +482,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             if (hit) {
                 // Packed neighbour: pool tet id (all ones at the mesh boundary) and region.
                 const uint32_t pk = imin == 0 ? L.x : (imin == 1 ? L.y : (imin == 2 ? L.z : L.w));
-                if ((pk & kLinkMask) != kLinkMask && (int)(pk >> kLinkBits) == region && -mincos >= kCos90F) {
+                // Same region: the region field equals the photon's and is not the escape
+                // (which the mesh boundary's all-ones word also carries).
+                const uint32_t field = pk >> kRegionShift;
+                if (field == (uint32_t)region && field != kRegionEscape && -mincos >= kCos90F) {
                     // FastReflectTransmit (src/octmps_kernel.cuh:590-805) on a face into the
                     // same region away from grazing incidence: equal indices give R = 0
                     // (sam == 0), so the photon transmits unless the ReflectT draw is 0,
                     // which it is exactly when the new MWC word is 0.
                     if (mwc_next(sR, aR) != 0u) {
                         tet = (int)(pk & kLinkMask);
+                        flags |= (((pk >> kBackShift) & 3u) + 1u) << kEntryShift;
                     } else {
                         const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
                         const float two_ca1 = -2.0f * mincos;
@@ -497,7 +528,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                 const float3 pf = make_float3(fr.x, fr.y, fr.z);
                 const uint32_t pk = __float_as_uint(fr.w);
                 const int nxt = (pk & kLinkMask) == kLinkMask ? -1 : (int)(pk & kLinkMask);
-                const int nr = (int)(pk >> kLinkBits);
+                const uint32_t field = pk >> kRegionShift;
+                // An escaped link carries no region: read it from the neighbour's record.
+                const int nr = (nxt < 0 || field != kRegionEscape) ? (int)field : A.tets[nxt].region;
                 // The same operations as the crossing phase's cosine of this face.
                 const float ca1 = -fmaf(pf.z, uz, fmaf(pf.y, uy, pf.x * ux));
                 // The empty asm keeps rn a register value: otherwise the three candidate
@@ -544,6 +577,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                     uz = rz;
                 } else if (rnd > R) {
                     tet = nxt;
+                    if (field != kRegionEscape) flags |= (((pk >> kBackShift) & 3u) + 1u) << kEntryShift;
                     if (nr != region) {
                         region = nr;
                         load_region(region);
