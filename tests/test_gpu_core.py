@@ -88,6 +88,59 @@ def test_layered_and_inclusion_parity_paired(ctx, table, name):
     assert c["R_rel"] < 0.005 and c["Rfilt_rel"] < 0.005
 
 
+def _layered_grid(n=(16, 16, 40), size=(0.2, 0.2, 0.1)):
+    """Cells split into 6 tets around the cell diagonal (conforming), region = layer k + 1."""
+    nx, ny, nz = n
+    xs = np.linspace(-size[0] / 2, size[0] / 2, nx + 1)
+    ys = np.linspace(-size[1] / 2, size[1] / 2, ny + 1)
+    zs = np.linspace(0.0, size[2], nz + 1)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    xyz = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+    def vid(i, j, k):
+        return (k * (ny + 1) + j) * (nx + 1) + i
+
+    tets = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                for order in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+                    c = [0, 0, 0]
+                    vs = [vid(i, j, k)]
+                    for a in order:
+                        c[a] += 1
+                        vs.append(vid(i + c[0], j + c[1], k + c[2]))
+                    tets.append(vs + [k + 1])
+    return xyz, np.array(tets, np.int32)
+
+
+def test_many_regions_escaped_links_parity_paired(ctx, table):
+    # 40 index-mismatched layers, 25 um thick (regions 1-40). A neighbour word has 5 bits
+    # for the neighbour's region, so links into regions >= 31 carry the escape value: the
+    # kernel then reads the region from the neighbour's record and fetches all four planes
+    # of the tet it enters. Photons reach those layers (the bottom quarter of the mesh).
+    n = 12000
+    xyz, tets = _layered_grid()
+    mesh = O.Mesh.build(xyz, tets)
+    regs = [configs._reg(1.33 + 0.004 * (i % 7), 0.05, 8.0, 0.9) for i in range(40)]
+    params = O.make_params(regs, num_bins=256, depth=0.06, **configs.BIAS)
+    root, z = mesh.find_root(0.0, 0.0, 0.0)
+    scan = O.ascan(0.0, 0.0, z, root, n)
+    gpu, st, plog, ref = _run_pair(ctx, table, mesh, params, scan, 1, threads=16)
+    assert st.photons == n and st.killed == 0 and ref["killed"] == 0
+    same_steps = np.mean(plog["steps"].astype(np.float64) == ref["log"][:, 0])
+    c = compare(gpu, ref, n)
+    print("escaped links", c, "same steps", same_steps)
+    assert same_steps > 0.5
+    assert c["frac_3sigma"] <= 0.01 and c["max_abs_z"] < 5
+    assert c["R_rel"] < 0.005 and c["Rfilt_rel"] < 0.005
+    mid = ctx.upload(mesh)
+    s2 = O.ascan(0.0, 0.0, z, root, 1500)
+    tb, _ = ctx.run_bscan(mid, params, [s2, s2], 7)
+    ta, _ = ctx.run_ascan(mid, params, s2, 7)
+    assert np.array_equal(ta, tb[0])
+
+
 @pytest.mark.parametrize("type_bias", [0, 3])
 def test_unbiased_and_no_spin_parity_paired(ctx, table, type_bias):
     # TypeBias 0 scatters with Spin (Henyey-Greenstein only) and accepts every photon that
