@@ -480,7 +480,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             if (maxz < z) maxz = z;
 
             if (hit) {
-                // Packed neighbour: pool tet id (all ones at the mesh boundary) and region.
+                // Packed neighbour word: pool tet id (all ones at the mesh boundary), the
+                // shared face's index within the neighbour, and the neighbour's region.
                 const uint32_t pk = imin == 0 ? L.x : (imin == 1 ? L.y : (imin == 2 ? L.z : L.w));
                 // Same region: the region field equals the photon's and is not the escape
                 // (which the mesh boundary's all-ones word also carries).
