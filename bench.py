@@ -27,6 +27,12 @@ LAYOUT_BYTES_PER_STEP = 80  # this build's packed tet record: 4 planes x 16 B + 
 # WRITE_SIZE (x1.0), divided by the launch's photon-steps. profiles/transport_slab4_pmc.md
 # holds the counter values and commands.
 TRAFFIC_B_PER_STEP = 26.23
+# Measured L1 lookups per photon-step of the batched slab4 launch (rocprofv3 --pmc
+# TCP_TOTAL_CACHE_ACCESSES_sum / photon-steps, profiles/transport_slab4_memory.md): the
+# 16-byte lane-loads of the tet record (5 per visit, 4 after entering through a face) plus
+# the other vector accesses. With octmps_probe_gather's ceiling for cached divergent record
+# gathers this gives the fraction of the gather path the kernel uses.
+LANE_LOADS_PER_STEP = 4.16
 
 
 def parse():
@@ -257,6 +263,18 @@ def main():
         print(f"[bench] served stream: {serving['requests']} requests, p50 {serving['p50_ms']:.1f} ms, "
               f"p99 {serving['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
 
+    # Ceiling of the vector-memory path for divergent record gathers (measured after the
+    # timed region, on a cache-resident record set).
+    gather = None
+    if a.workload == "slab4":
+        peak = ctx.probe_gather()
+        got = sum(steps) / (sum(kms) * 1e-3) * LANE_LOADS_PER_STEP
+        gather = {"bound": "vector-memory gathers", "achieved": got, "peak": peak, "unit": "16-byte lane-loads/s",
+                  "frac": got / peak, "lane_loads_per_step": LANE_LOADS_PER_STEP,
+                  "peak_source": "octmps_probe_gather: 5 x 16 B from pseudo-random 128-byte records of a "
+                                 "2 MiB set, 6 waves per SIMD"}
+        print(f"[bench] gather ceiling {peak:.3e} lane-loads/s; kernel {got:.3e} ({got / peak:.2f})",
+              file=sys.stderr, flush=True)
     ctx.close()
     line = {
         "metric": "photons/s per B-scan, batched vs plain kernel; p50/p99 A-scan request latency",
@@ -284,6 +302,7 @@ def main():
                      "traffic": (steps_per_launch * TRAFFIC_B_PER_STEP) if TRAFFIC_B_PER_STEP else None,
                      "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step",
                      "layout_bytes_per_step": LAYOUT_BYTES_PER_STEP},
+        "gather_roofline": gather,
         "cpu_baseline": cpu,
         "serving": serving,
     }

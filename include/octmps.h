@@ -342,6 +342,12 @@ octmps_status octmps_probe_sampler(octmps_ctx *ctx, int32_t kind, float param, i
  * reference): the calibration load for hardware byte counters. Returns the kernel time. */
 octmps_status octmps_probe_copy(octmps_ctx *ctx, uint64_t bytes, int32_t repeats, double *ms);
 
+/* Divergent gathers of 5 x 16 B from pseudo-random 128-byte records of an n_records set
+ * that stays in cache, 6 waves per SIMD (no counterpart in the reference): the ceiling of
+ * the vector-memory path for the access pattern of a photon-step's tet visit. Returns 16-byte
+ * lane-loads per second. n_records in 1..2^24. */
+octmps_status octmps_probe_gather(octmps_ctx *ctx, int32_t n_records, int32_t iters, double *lane_loads_per_s);
+
 #ifdef __cplusplus
 }
 #endif

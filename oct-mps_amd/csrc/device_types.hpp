@@ -182,5 +182,8 @@ hipError_t launch_probe_fresnel(int n, const float *ni, const float *nt, const f
 hipError_t launch_probe_sampler(int kind, float param, int n, const float *u, float *out,
                                 hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
+// blocks x 256 lanes each read iters records (5 x 16 B) of recs[0..n).
+hipError_t launch_gather_probe(const TetRec *recs, uint32_t n, int iters, uint32_t *out, int blocks,
+                               hipStream_t stream);
 
 }  // namespace octmps

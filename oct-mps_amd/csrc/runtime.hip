@@ -603,6 +603,36 @@ extern "C" octmps_status octmps_probe_sampler(octmps_ctx *c, int32_t kind, float
     return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_sampler");
 }
 
+extern "C" octmps_status octmps_probe_gather(octmps_ctx *c, int32_t n_records, int32_t iters,
+                                            double *lane_loads_per_s) {
+    if (!c || n_records <= 0 || n_records > (1 << 24) || iters <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
+    std::lock_guard<std::mutex> lk(c->run_mu);
+    hipSetDevice(c->device);
+    const int blocks = c->n_cu * 6;  // 6 waves per SIMD, as the transport kernel
+    void *recs = nullptr, *out = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&recs, (size_t)n_records * sizeof(TetRec))) != hipSuccess) return hip_fail(e, "probe allocation");
+    if ((e = hipMalloc(&out, (size_t)blocks * 256 * sizeof(uint32_t))) != hipSuccess) {
+        (void)hipFree(recs);
+        return hip_fail(e, "probe allocation");
+    }
+    e = hipMemsetAsync(recs, 1, (size_t)n_records * sizeof(TetRec), c->stream);
+    if (e == hipSuccess)
+        e = launch_gather_probe((const TetRec *)recs, (uint32_t)n_records, iters, (uint32_t *)out, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess)
+        e = launch_gather_probe((const TetRec *)recs, (uint32_t)n_records, iters, (uint32_t *)out, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    float ms = 0;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(recs);
+    (void)hipFree(out);
+    if (lane_loads_per_s) *lane_loads_per_s = ms > 0 ? (double)blocks * 256.0 * iters * 5.0 / (ms * 1e-3) : 0.0;
+    return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_gather");
+}
+
 extern "C" octmps_status octmps_probe_copy(octmps_ctx *c, uint64_t bytes, int32_t repeats, double *ms) {
     if (!c || bytes < 16 || repeats <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
     std::lock_guard<std::mutex> lk(c->run_mu);

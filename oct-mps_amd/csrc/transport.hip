@@ -849,6 +849,37 @@ __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ d
 }
 }  // namespace
 
+namespace {
+// Divergent 16-byte gathers from 128-byte records that stay in cache: each lane visits
+// pseudo-random records of a small set and reads 5 x 16 B of each, as a photon-step reads
+// its tet record. Every offset is (index mod n) * 128 + 16 * j with j < 5, inside the
+// n * 128 bytes of the buffer.
+__global__ void __launch_bounds__(256) gather_probe_kernel(const TetRec *recs, uint32_t n, int iters,
+                                                           uint32_t *out) {
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void *)recs, (short)0, (int)(n * 128u), kBufferWord3);
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t idx = g * 2654435761u, acc = 0;
+    for (int i = 0; i < iters; ++i) {
+        idx = idx * 1664525u + 1013904223u;
+        const uint32_t rec = (idx % n) * 128u;
+        const u4v a = __builtin_amdgcn_raw_buffer_load_b128(r, rec, 0, 0);
+        const u4v b = __builtin_amdgcn_raw_buffer_load_b128(r, rec, 16, 0);
+        const u4v c = __builtin_amdgcn_raw_buffer_load_b128(r, rec, 32, 0);
+        const u4v d = __builtin_amdgcn_raw_buffer_load_b128(r, rec, 48, 0);
+        const u4v e = __builtin_amdgcn_raw_buffer_load_b128(r, rec, 64, 0);
+        acc += a.x ^ b.y ^ c.z ^ d.w ^ e.x;
+    }
+    out[g] = acc;
+}
+}  // namespace
+
+hipError_t launch_gather_probe(const TetRec *recs, uint32_t n, int iters, uint32_t *out, int blocks,
+                               hipStream_t stream) {
+    hipLaunchKernelGGL(gather_probe_kernel, dim3(blocks), dim3(256), 0, stream, recs, n, iters, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream) {
     uint64_t n = bytes / sizeof(uint4);
     hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, stream, (const uint4 *)src, (uint4 *)dst, n);

@@ -181,6 +181,7 @@ SIGNATURES = [
     ("octmps_probe_sampler", C.c_int, [_P, C.c_int32, C.c_float, C.c_int32, C.POINTER(C.c_float),
                                        C.POINTER(C.c_float)]),
     ("octmps_probe_copy", C.c_int, [_P, C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
+    ("octmps_probe_gather", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
 ]
 
 
@@ -441,6 +442,12 @@ class Context:
         ms = C.c_double(0)
         check(lib.octmps_probe_copy(self.h, nbytes, repeats, C.byref(ms)), "probe_copy")
         return ms.value
+
+    def probe_gather(self, n_records=1 << 14, iters=4000):
+        """16-byte lane-loads per second of cached divergent record gathers (5 per record)."""
+        r = C.c_double(0)
+        check(lib.octmps_probe_gather(self.h, n_records, iters, C.byref(r)), "probe_gather")
+        return r.value
 
     def probe_sampler(self, kind, param, u):
         u = np.ascontiguousarray(u, np.float32)

@@ -139,6 +139,14 @@ def test_translation_invariance_in_layers(ctx):
     assert abs(ra - rb) < 5 * sr
 
 
+def test_gather_probe_measures_a_sane_ceiling(ctx):
+    # 256 CUs issue at most a few 16-byte lane-loads per cycle between them at ~2.4 GHz.
+    r = ctx.probe_gather(1 << 12, 200)
+    assert 1e10 < r < 1e13
+    with pytest.raises(O.OctmpsError):
+        ctx.probe_gather(0, 10)
+
+
 def test_autotune_picks_a_valid_configuration(ctx, small_meshes):
     _, mid, p, s, seed = small_meshes["slab"]
     o, rate = ctx.autotune(mid, p, s, seed, calib=4000)
