@@ -168,7 +168,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     std::shared_lock<std::shared_mutex> pk(ctx->pool_mu);
     if (mesh_id < 0 || mesh_id >= (int)ctx->meshes.size() || !ctx->meshes[mesh_id].live)
         return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
-    hipSetDevice(ctx->device);
+    if (hipError_t se = hipSetDevice(ctx->device); se != hipSuccess) return hip_fail(se, "set device");
     const MeshEntry &me = ctx->meshes[mesh_id];
     DevSet S;
     std::vector<DevRegion> R(kMaxRegions);
@@ -356,15 +356,15 @@ extern "C" octmps_status octmps_ctx_create(int32_t device, octmps_ctx **out) {
 
 extern "C" void octmps_ctx_destroy(octmps_ctx *c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->graph.exec) hipGraphExecDestroy(c->graph.exec);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
     c->work.release_all();
-    if (c->pool_tets) hipFree(c->pool_tets);
-    if (c->d_mult) hipFree(c->d_mult);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->pool_tets) (void)hipFree(c->pool_tets);
+    if (c->d_mult) (void)hipFree(c->d_mult);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -374,7 +374,7 @@ extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh
     const int64_t T = m.ntets();
     if (m.max_region >= kMaxRegions) return fail(OCTMPS_E_RANGE, "regions must be < 100");
     std::unique_lock<std::shared_mutex> pk(c->pool_mu);
-    hipSetDevice(c->device);
+    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
     if (c->pool_used + T > kMaxPoolTets)
         return fail(OCTMPS_E_RANGE, "resident meshes exceed 2^25 - 1 tets (packed neighbour ids)");
     hipError_t e;
@@ -383,13 +383,17 @@ extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh
         TetRec *np = nullptr;
         if ((e = hipMalloc(&np, (size_t)ncap * sizeof(TetRec))) != hipSuccess)
             return fail(OCTMPS_E_NOMEM, "mesh pool allocation failed");
-        if (c->pool_used > 0)
-            hipMemcpy(np, c->pool_tets, (size_t)c->pool_used * sizeof(TetRec), hipMemcpyDeviceToDevice);
-        if (c->pool_tets) hipFree(c->pool_tets);
+        if (c->pool_used > 0 &&
+            (e = hipMemcpy(np, c->pool_tets, (size_t)c->pool_used * sizeof(TetRec), hipMemcpyDeviceToDevice)) !=
+                hipSuccess) {
+            (void)hipFree(np);
+            return hip_fail(e, "mesh pool growth");
+        }
+        if (c->pool_tets) (void)hipFree(c->pool_tets);
         c->pool_tets = np;
         c->pool_cap = ncap;
         if (c->graph.exec) {
-            hipGraphExecDestroy(c->graph.exec);
+            (void)hipGraphExecDestroy(c->graph.exec);
             c->graph.exec = nullptr;
         }
     }
@@ -557,7 +561,7 @@ extern "C" octmps_status octmps_probe_rng(octmps_ctx *c, uint64_t seed, uint64_t
                                           int32_t n_photons, int32_t n_draws, float *out) {
     if (!c || !out || n_photons <= 0 || n_draws <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
     std::lock_guard<std::mutex> lk(c->run_mu);
-    hipSetDevice(c->device);
+    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
     size_t nb = (size_t)n_photons * 3 * n_draws * sizeof(float);
     float *d = nullptr;
     hipError_t e;
@@ -565,7 +569,7 @@ extern "C" octmps_status octmps_probe_rng(octmps_ctx *c, uint64_t seed, uint64_t
     e = launch_probe_rng(c->d_mult, c->table->n_triplets, seed, salt, photon0, n_photons, n_draws, d, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out, d, nb, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d);
+    (void)hipFree(d);
     return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_rng");
 }
 
@@ -573,17 +577,17 @@ extern "C" octmps_status octmps_probe_fresnel(octmps_ctx *c, int32_t n, const fl
                                               const float *ca1, float *r) {
     if (!c || n <= 0 || !ni || !nt || !ca1 || !r) return fail(OCTMPS_E_INVALID, "bad probe arguments");
     std::lock_guard<std::mutex> lk(c->run_mu);
-    hipSetDevice(c->device);
+    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
     float *d = nullptr;
     hipError_t e;
     if ((e = hipMalloc(&d, 4 * (size_t)n * sizeof(float))) != hipSuccess) return hip_fail(e, "probe allocation");
-    hipMemcpyAsync(d, ni, n * 4, hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(d + n, nt, n * 4, hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(d + 2 * (size_t)n, ca1, n * 4, hipMemcpyHostToDevice, c->stream);
-    e = launch_probe_fresnel(n, d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, c->stream);
+    e = hipMemcpyAsync(d, ni, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + n, nt, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + 2 * (size_t)n, ca1, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_probe_fresnel(n, d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r, d + 3 * (size_t)n, n * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d);
+    (void)hipFree(d);
     return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_fresnel");
 }
 
@@ -591,15 +595,15 @@ extern "C" octmps_status octmps_probe_sampler(octmps_ctx *c, int32_t kind, float
                                               float *out) {
     if (!c || n <= 0 || !u || !out || (kind != 0 && kind != 1)) return fail(OCTMPS_E_INVALID, "bad probe arguments");
     std::lock_guard<std::mutex> lk(c->run_mu);
-    hipSetDevice(c->device);
+    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
     float *d = nullptr;
     hipError_t e;
     if ((e = hipMalloc(&d, 2 * (size_t)n * sizeof(float))) != hipSuccess) return hip_fail(e, "probe allocation");
-    hipMemcpyAsync(d, u, n * 4, hipMemcpyHostToDevice, c->stream);
-    e = launch_probe_sampler(kind, param, n, d, d + n, c->stream);
+    e = hipMemcpyAsync(d, u, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_probe_sampler(kind, param, n, d, d + n, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out, d + n, n * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d);
+    (void)hipFree(d);
     return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_sampler");
 }
 
@@ -607,7 +611,7 @@ extern "C" octmps_status octmps_probe_gather(octmps_ctx *c, int32_t n_records, i
                                             double *lane_loads_per_s) {
     if (!c || n_records <= 0 || n_records > (1 << 24) || iters <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
     std::lock_guard<std::mutex> lk(c->run_mu);
-    hipSetDevice(c->device);
+    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
     const int blocks = c->n_cu * 6;  // 6 waves per SIMD, as the transport kernel
     void *recs = nullptr, *out = nullptr;
     hipError_t e;
@@ -636,7 +640,7 @@ extern "C" octmps_status octmps_probe_gather(octmps_ctx *c, int32_t n_records, i
 extern "C" octmps_status octmps_probe_copy(octmps_ctx *c, uint64_t bytes, int32_t repeats, double *ms) {
     if (!c || bytes < 16 || repeats <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
     std::lock_guard<std::mutex> lk(c->run_mu);
-    hipSetDevice(c->device);
+    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
     void *a = nullptr, *b = nullptr;
     hipError_t e;
     if ((e = hipMalloc(&a, bytes)) != hipSuccess) return hip_fail(e, "probe allocation");

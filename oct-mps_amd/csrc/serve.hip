@@ -204,7 +204,7 @@ void finish(octmps_server *s, Req *r, int status) {
 
 void scheduler(octmps_server *s) {
     octmps_ctx *ctx = s->ctx;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     std::vector<uint64_t> done_h((size_t)s->n_slots);
     for (;;) {
         std::vector<Req *> staged;
@@ -326,7 +326,7 @@ void scheduler(octmps_server *s) {
         }
         s->have_saved = true;
         float kms = 0;
-        hipEventElapsedTime(&kms, s->e0, s->e1);
+        (void)hipEventElapsedTime(&kms, s->e0, s->e1);
         s->st_kernel_ms += kms;
         s->st_steps += ctrl[CTRL_STEPS];
         s->st_slices++;
@@ -388,7 +388,10 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
         delete s;
         return fail(OCTMPS_E_INVALID, "block size must be 64, 128, 192 or 256");
     }
-    hipSetDevice(ctx->device);
+    if (hipError_t se = hipSetDevice(ctx->device); se != hipSuccess) {
+        delete s;
+        return hip_fail(se, "set device");
+    }
     grid_for(ctx, s->block, 0, true, s->grid);
     s->lanes = (uint64_t)s->grid * s->block;
     uint64_t budget = s->o.hbm_budget_bytes ? s->o.hbm_budget_bytes : (1ull << 30);
@@ -536,14 +539,14 @@ extern "C" octmps_status octmps_serve_close(octmps_server *s) {
         s->cv_work.notify_all();
     }
     if (s->th.joinable()) s->th.join();
-    hipSetDevice(s->ctx->device);
-    if (s->stream) hipStreamSynchronize(s->stream);
+    (void)hipSetDevice(s->ctx->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (DevBuf *b : {&s->regions, &s->sets, &s->slots, &s->seg_start, &s->seg_slot, &s->seg_poff, &s->tallies,
                       &s->ctrl, &s->slot_done, &s->save, &s->cont})
         b->release();
-    if (s->e0) hipEventDestroy(s->e0);
-    if (s->e1) hipEventDestroy(s->e1);
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->e0) (void)hipEventDestroy(s->e0);
+    if (s->e1) (void)hipEventDestroy(s->e1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     for (auto &kv : s->reqs) delete kv.second;
     delete s;
     return OCTMPS_OK;
