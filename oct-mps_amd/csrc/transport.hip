@@ -206,8 +206,9 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Six waves per SIMD (<= 80 VGPRs, no spills): the loop is bound by dependent gathers of
-// tet records, so more resident waves hide more of their latency.
+// Six waves per SIMD (<= 80 VGPRs, no spills). The loop is bound by the per-lane cost of its
+// tet-record gathers rather than by their latency: five waves per SIMD run as fast, and
+// seven are slower (more photons in flight per CU thrash the L1).
 template <bool MULTI, bool LOG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) transport_kernel(KArgs A) {
     const uint32_t lane = __lane_id();
