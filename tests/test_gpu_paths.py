@@ -139,6 +139,42 @@ def test_translation_invariance_in_layers(ctx):
     assert abs(ra - rb) < 5 * sr
 
 
+def test_zero_budget_a_scan_in_a_batch(ctx, small_meshes):
+    # An A-scan with no photons gets an all-zero tally and changes nothing for the others
+    # (every photon's streams depend only on its own A-scan's seed, salt and index).
+    m, mid, p, scans, seed = small_meshes["slab"]
+    ref, _ = ctx.run_bscan(mid, p, scans, seed)
+    s2 = scans[2]
+    mixed = list(scans[:2]) + [O.ascan(s2.x, s2.y, s2.z, s2.root_tet, 0, s2.stream_salt)] + list(scans[3:])
+    got, st = ctx.run_bscan(mid, p, mixed, seed)
+    assert st.photons == sum(s.n_photons for s in mixed)
+    assert not got[2].any()
+    for i in (0, 1, 3):
+        assert np.array_equal(got[i], ref[i]), i
+
+
+def test_step_guard_kills_the_photons_the_oracle_kills(ctx, table):
+    # max_steps_per_photon ends a photon after that many steps, as the oracle's max_steps
+    # does; with paired streams the same photons are cut short (a few may differ where an
+    # FP32 and an FP64 step decide a near-edge crossing differently).
+    import oracle_py as OR
+    n, cap = 3000, 3000
+    mesh, params, scans, seed = configs.sphere(cells=12, photons=n, bins=96)
+    mid = ctx.upload(mesh)
+    o = O.BatchOpts()
+    o.max_steps_per_photon = cap
+    t, st, plog = ctx.run_bscan(mid, params, scans, seed, opts=o, log=True)
+    ref = OR.run_ascan(mesh.soa(), params, scans[0], seed, table, threads=8, max_steps=cap, log=True)
+    assert st.photons == n and ref["killed"] > n // 2
+    gpu_killed = plog["killed"].astype(bool)
+    # The oracle's log has no kill flag; a photon that reached the cap was killed, except
+    # one whose last step (a roulette death) was exactly the cap-th.
+    ref_killed = ref["log"][:, 0] >= cap
+    assert st.killed == int(gpu_killed.sum()) and 0 <= int(ref_killed.sum()) - ref["killed"] <= 5
+    assert np.mean(gpu_killed == ref_killed) > 0.95
+    assert abs(st.killed - ref["killed"]) <= 0.03 * ref["killed"]
+
+
 def test_gather_probe_measures_a_sane_ceiling(ctx):
     # 256 CUs issue at most a few 16-byte lane-loads per cycle between them at ~2.4 GHz.
     r = ctx.probe_gather(1 << 12, 200)
