@@ -245,7 +245,10 @@ typedef struct {
     int32_t tally_copies;    /* tally privatisation depth: private copies of every A-scan's
                                 tally (workgroup w adds into copy w mod k), folded into one
                                 after the launch; 1..16, 0 = autotuned/default (1) */
-    int32_t reserved;        /* must be 0 */
+    int32_t entry_skip;      /* a photon that crossed into a tet through face k does not fetch
+                                plane k: 0 = auto (on when the mesh's 128-byte tet records fit
+                                the 256 MB Infinity Cache), 1 = on, 2 = off. Results are the
+                                same bits either way. */
     uint64_t max_steps_per_photon; /* 0 = default guard (1<<22) */
 } octmps_batch_opts;
 
@@ -272,10 +275,11 @@ octmps_status octmps_run_bscan_logged(octmps_ctx *ctx, int32_t mesh_id, const oc
                                       const octmps_batch_opts *opts, uint64_t *tallies_out,
                                       octmps_run_stats *stats, void *photon_log,
                                       uint64_t log_records);
-/* Autotune scatter quantum, block size, refill quantum and tally privatisation depth on a
- * calibration slice of a B-scan (coordinate search, 14 launches); writes the winner into
- * *opts (scatter_quantum, block_size, refill_quantum, tally_copies), makes it the context's
- * default, and returns its photons/s in *rate. */
+/* Autotune scatter quantum, block size, refill quantum, tally privatisation depth and the
+ * entry-plane skip on a calibration slice of a B-scan (coordinate search, 16 launches);
+ * writes the winner into *opts (scatter_quantum, block_size, refill_quantum, tally_copies,
+ * entry_skip), makes the first four the context's default (entry_skip stays per launch, since
+ * its best value depends on the mesh), and returns its photons/s in *rate. */
 octmps_status octmps_autotune(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p,
                               int32_t n_scans, const octmps_ascan *scans, uint64_t seed,
                               uint64_t calib_photons, octmps_batch_opts *opts, double *rate);

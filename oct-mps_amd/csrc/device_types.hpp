@@ -135,6 +135,7 @@ enum { CTRL_NEXT_ID = 0, CTRL_STEPS = 1, CTRL_KILLED = 2, CTRL_DONE = 3, CTRL_IT
 struct KArgs {
     const TetRec *tets;
     uint32_t tets_bytes;        // bytes of resident records (<= (2^25 - 1) * 128 < 2^32)
+    uint32_t skip_entry;        // 1: the plane of a tet's entry face is not fetched
     const DevRegion *regions;
     const DevSet *sets;
     const DevScan *slots;

@@ -200,8 +200,9 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     if (block % 64 != 0 || block > 256) return fail(OCTMPS_E_INVALID, "block size must be 64, 128, 192 or 256");
     const int q = (opts && opts->refill_quantum > 0) ? opts->refill_quantum : default_quantum(ctx);
     const int copies = (opts && opts->tally_copies > 0) ? opts->tally_copies : default_copies(ctx);
-    if (copies > 16 || (opts && opts->reserved != 0))
-        return fail(OCTMPS_E_INVALID, "tally_copies must be 1..16 and reserved 0");
+    if (copies > 16) return fail(OCTMPS_E_INVALID, "tally_copies must be 1..16");
+    const int skip_opt = opts ? opts->entry_skip : 0;
+    if (skip_opt < 0 || skip_opt > 2) return fail(OCTMPS_E_INVALID, "entry_skip must be 0, 1 or 2");
     int grid = 0;
     grid_for(ctx, block, opts ? opts->blocks_per_cu : 0, false, grid);
     hipError_t e;
@@ -231,6 +232,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     std::memset(&A, 0, sizeof A);
     A.tets = ctx->pool_tets;
     A.tets_bytes = (uint32_t)(ctx->pool_used * (int64_t)sizeof(TetRec));
+    A.skip_entry = skip_opt == 1 || (skip_opt == 0 && entry_skip_fits(me.n_tets));
     A.regions = (const DevRegion *)W.regions.p;
     A.sets = (const DevSet *)W.sets.p;
     A.slots = (const DevScan *)W.slots.p;
@@ -503,15 +505,15 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     for (auto &s : cs) s.n_photons = std::min<uint64_t>(s.n_photons, per);
     std::vector<uint64_t> t((size_t)n * tally_words(p->num_depth_bins));
     // Coordinate search: scatter quantum, then block size, then refill quantum, then tally
-    // privatisation depth, each with the best values found so far for the others (14
-    // calibration launches).
+    // privatisation depth, then the entry-plane skip, each with the best values found so far
+    // for the others (16 calibration launches).
     double best = -1;
     octmps_batch_opts win = *opts;
     win.block_size = 256;
     win.refill_quantum = 16;
     win.scatter_quantum = 40;
     win.tally_copies = 1;
-    win.reserved = 0;
+    win.entry_skip = 0;
     win.use_graph = 0;
     auto trial = [&](octmps_batch_opts o) -> octmps_status {
         octmps_run_stats st;
@@ -545,10 +547,16 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
         o.tally_copies = k;
         if ((s = trial(o)) != OCTMPS_OK) return s;
     }
+    for (int sk : {1, 2}) {
+        octmps_batch_opts o = win;
+        o.entry_skip = sk;
+        if ((s = trial(o)) != OCTMPS_OK) return s;
+    }
     opts->block_size = win.block_size;
     opts->refill_quantum = win.refill_quantum;
     opts->scatter_quantum = win.scatter_quantum;
     opts->tally_copies = win.tally_copies;
+    opts->entry_skip = win.entry_skip;
     c->tuned_block = win.block_size;
     c->tuned_q = win.refill_quantum;
     c->tuned_sq = win.scatter_quantum;

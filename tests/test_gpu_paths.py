@@ -50,8 +50,21 @@ def test_tally_privatisation_depth_never_changes_results(ctx, small_meshes, copi
         assert np.array_equal(ref, got)
 
 
-@pytest.mark.parametrize("field,value", [("tally_copies", 17), ("reserved", 1), ("block_size", 96),
-                                         ("block_size", 512)])
+@pytest.mark.parametrize("entry_skip", [1, 2])
+def test_entry_plane_skip_never_changes_results(ctx, small_meshes, entry_skip):
+    # Skipping the plane of the face a photon entered through removes a face that cannot be
+    # its exit: the same bits with the skip forced on or off, as with the default.
+    for name in ("slab", "ellipsoid"):
+        m, mid, p, scans, seed = small_meshes[name]
+        ref, _ = ctx.run_bscan(mid, p, scans, seed)
+        o = O.BatchOpts()
+        o.entry_skip = entry_skip
+        got, _ = ctx.run_bscan(mid, p, scans, seed, opts=o)
+        assert np.array_equal(ref, got), name
+
+
+@pytest.mark.parametrize("field,value", [("tally_copies", 17), ("entry_skip", 3), ("entry_skip", -1),
+                                         ("block_size", 96), ("block_size", 512)])
 def test_invalid_launch_options_are_rejected(ctx, small_meshes, field, value):
     m, mid, p, scans, seed = small_meshes["ellipsoid"]
     o = O.BatchOpts()
@@ -188,6 +201,7 @@ def test_autotune_picks_a_valid_configuration(ctx, small_meshes):
     o, rate = ctx.autotune(mid, p, s, seed, calib=4000)
     assert o.block_size in (64, 128, 256) and o.refill_quantum in (8, 12, 16, 24) and rate > 0
     assert o.scatter_quantum in (32, 36, 40, 44) and o.tally_copies in (1, 2, 4, 8)
+    assert o.entry_skip in (0, 1, 2)
     t1 = ctx.run_bscan(mid, p, s, seed)[0]
     t2 = ctx.run_bscan(mid, p, s, seed, opts=o)[0]
     assert np.array_equal(t1, t2)  # launch shape never changes results
