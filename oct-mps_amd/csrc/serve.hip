@@ -15,6 +15,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -124,7 +125,12 @@ void free_tally(octmps_server *s, uint64_t off, uint64_t words) {
 void release(octmps_server *s, Req *r) {
     for (ScanState &x : r->ss)
         if (x.slot >= 0) s->free_slots.push_back(x.slot);
-    if (r->set >= 0) s->free_sets.push_back(r->set);
+    if (r->set >= 0) {
+        // Smallest id first, so that up to kLdsSets requests in flight keep their
+        // parameter sets in LDS.
+        s->free_sets.push_back(r->set);
+        std::sort(s->free_sets.begin(), s->free_sets.end(), std::greater<int>());
+    }
     if (r->admitted) free_tally(s, r->tally_base, r->words * r->scans.size());
     r->set = -1;
     r->admitted = false;
@@ -227,11 +233,17 @@ void scheduler(octmps_server *s) {
         std::vector<uint32_t> seg_slot;
         std::vector<std::pair<Req *, int>> seg_owner;
         std::vector<std::pair<Req *, int>> open;
+        // Every live photon belongs to an active request (a request ends only once all
+        // photons it handed out are done), so the sets of the active requests are the
+        // only ones the slice can touch.
+        bool sets_in_lds = true;
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            for (Req *r : s->active)
+            for (Req *r : s->active) {
+                if (r->set >= kLdsSets || ctx->meshes[r->mesh_id].max_region >= kLdsSetRegions) sets_in_lds = false;
                 for (size_t i = 0; i < r->ss.size(); ++i)
                     if (r->ss[i].claimed < r->ss[i].limit) open.push_back({r, (int)i});
+            }
         }
         uint64_t budget = s->o.slice_photons ? s->o.slice_photons : s->lanes * 4;
         uint64_t total = 0;
@@ -302,6 +314,7 @@ void scheduler(octmps_server *s) {
         A.slice_mode = 1;
         A.restore = s->have_saved ? 1u : 0u;
         A.exit_on_drain = total > 0 ? 1u : 0u;
+        A.sets_in_lds = sets_in_lds ? 1u : 0u;
         uint64_t ctrl[CTRL_WORDS] = {0};
         if (ok) ok = (e = hipEventRecord(s->e0, s->stream)) == hipSuccess &&
                      (e = launch_transport(A, s->grid, s->block, true, false, s->stream)) == hipSuccess &&

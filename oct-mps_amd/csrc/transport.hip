@@ -214,7 +214,11 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
 // when many records miss the caches (an 805 MB mesh ran 30 % faster than with SKIP, and 50 %
 // faster than with the same offsets computed per plane); with SKIP the smaller gather wins
 // when records come from cache (+2 % on a 101 MB mesh). The launch picks one per mesh.
-template <bool MULTI, bool LOG, bool SKIP>
+// SETS_LDS (MULTI only): the parameter sets 0..kLdsSets-1 and their regions
+// 0..kLdsSetRegions-1 are copied to LDS at launch, and every lane's set must be among them;
+// otherwise each lane reads its set's records from memory, a per-lane gather in the parked
+// phase that made serving 25 % slower than a single-set launch.
+template <bool MULTI, bool LOG, bool SKIP, bool SETS_LDS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) transport_kernel(KArgs A) {
     const uint32_t lane = __lane_id();
     const uint64_t gl = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -240,11 +244,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
 
     // One parameter set per launch unless MULTI: read it once (uniform, scalar registers),
     // and keep its region table in LDS, since a photon changes region on a large share of
-    // its steps in layered tissue. MULTI reads the table of each lane's set from memory.
+    // its steps in layered tissue. MULTI with SETS_LDS keeps the first kLdsSets sets and
+    // their region tables in LDS; MULTI without it reads each lane's set from memory.
     const DevSet S0 = A.sets[0];
-    __shared__ DevRegion lds_regions[MULTI ? 1 : kMaxRegions];
+    constexpr int kRegionSlots = MULTI ? (SETS_LDS ? kLdsSets * kLdsSetRegions : 1) : kMaxRegions;
+    __shared__ DevRegion lds_regions[kRegionSlots];
+    __shared__ DevSet lds_sets[MULTI && SETS_LDS ? kLdsSets : 1];
     if (!MULTI) {
         for (int i = threadIdx.x; i < kMaxRegions; i += blockDim.x) lds_regions[i] = A.regions[S0.region_base + i];
+        __syncthreads();
+    } else if (SETS_LDS) {
+        for (int i = threadIdx.x; i < kLdsSets * kLdsSetRegions; i += blockDim.x)
+            lds_regions[i] = A.regions[(i / kLdsSetRegions) * kMaxRegions + i % kLdsSetRegions];
+        for (int i = threadIdx.x; i < kLdsSets; i += blockDim.x) lds_sets[i] = A.sets[i];
         __syncthreads();
     }
     // Per-lane LDS stash (blocks have at most 256 lanes): the probe position of the lane's
@@ -254,7 +266,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     __shared__ float4 lds_probe[256], lds_face[256];
     auto region_rec = [&](int r) -> const DevRegion & {
         // The region table of set s starts at s * kMaxRegions (DevSet::region_base).
-        return MULTI ? A.regions[set * kMaxRegions + r] : lds_regions[r];
+        return MULTI ? (SETS_LDS ? lds_regions[set * kLdsSetRegions + r] : A.regions[set * kMaxRegions + r])
+                     : lds_regions[r];
     };
     auto load_region = [&](int r) {
         const DevRegion &R = region_rec(r);
@@ -379,7 +392,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
         const bool ready = (flags & (F_ALIVE | F_PEND | F_IFACE)) == F_ALIVE;
         wave_steps += (uint64_t)__popcll(__ballot(ready));
 
-        const DevSet &S = MULTI ? A.sets[set] : S0;
+        const DevSet &S = MULTI ? (SETS_LDS ? lds_sets[set] : A.sets[set]) : S0;
         bool finish = false, killed = false;
 
         // Roulette, src/octmps_kernel.cuh:1376-1392; returns true when the photon is done.
@@ -766,16 +779,32 @@ namespace {
 // variant never skips (its register budget is full: with the skip it spills 14 registers).
 template <bool LOG>
 void launch_single(const KArgs &a, int grid, int block, hipStream_t stream) {
-    if (a.skip_entry) hipLaunchKernelGGL((transport_kernel<false, LOG, true>), dim3(grid), dim3(block), 0, stream, a);
-    else hipLaunchKernelGGL((transport_kernel<false, LOG, false>), dim3(grid), dim3(block), 0, stream, a);
+    if (a.skip_entry) hipLaunchKernelGGL((transport_kernel<false, LOG, true, false>), dim3(grid), dim3(block), 0, stream, a);
+    else hipLaunchKernelGGL((transport_kernel<false, LOG, false, false>), dim3(grid), dim3(block), 0, stream, a);
+}
+
+// Multi-set variant with the parameter sets in LDS when the launch allows it.
+template <bool LOG>
+void launch_multi(const KArgs &a, int grid, int block, hipStream_t stream) {
+    if (a.sets_in_lds) hipLaunchKernelGGL((transport_kernel<true, LOG, false, true>), dim3(grid), dim3(block), 0, stream, a);
+    else hipLaunchKernelGGL((transport_kernel<true, LOG, false, false>), dim3(grid), dim3(block), 0, stream, a);
+}
+
+template <bool LOG>
+int occupancy_multi(int block) {
+    int a = 0, b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, transport_kernel<true, LOG, false, true>, block, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, transport_kernel<true, LOG, false, false>, block, 0) != hipSuccess)
+        return 0;
+    return a < b ? a : b;
 }
 
 // Blocks per CU of the tighter of the two single-set variants.
 template <bool LOG>
 int occupancy_single(int block) {
     int a = 0, b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, transport_kernel<false, LOG, true>, block, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, transport_kernel<false, LOG, false>, block, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, transport_kernel<false, LOG, true, false>, block, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, transport_kernel<false, LOG, false, false>, block, 0) != hipSuccess)
         return 0;
     return a < b ? a : b;
 }
@@ -783,8 +812,8 @@ int occupancy_single(int block) {
 
 hipError_t launch_transport(const KArgs &a, int grid, int block, bool multi, bool log, hipStream_t stream) {
     if (multi) {
-        if (log) hipLaunchKernelGGL((transport_kernel<true, true, false>), dim3(grid), dim3(block), 0, stream, a);
-        else hipLaunchKernelGGL((transport_kernel<true, false, false>), dim3(grid), dim3(block), 0, stream, a);
+        if (log) launch_multi<true>(a, grid, block, stream);
+        else launch_multi<false>(a, grid, block, stream);
     } else {
         if (log) launch_single<true>(a, grid, block, stream);
         else launch_single<false>(a, grid, block, stream);
@@ -796,9 +825,8 @@ int transport_blocks_per_cu(int block, bool multi, bool log) {
     int n = 0;
     hipError_t e;
     if (multi) {
-        e = log ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<true, true, false>, block, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<true, false, false>, block, 0);
-        return e == hipSuccess ? n : 0;
+        (void)e;
+        return log ? occupancy_multi<true>(block) : occupancy_multi<false>(block);
     }
     return log ? occupancy_single<true>(block) : occupancy_single<false>(block);
 }

@@ -95,6 +95,47 @@ def test_served_results_equal_batched_alone_and_in_a_mix(ctx, small_meshes):
         srv.close()
 
 
+def test_served_sets_beyond_the_lds_cache_equal_batched(ctx, small_meshes):
+    # A slice keeps the parameter sets in LDS only while every request in flight has a set
+    # id below 16 and a mesh whose regions are below 16; otherwise the kernel reads each
+    # lane's set from memory. Both give the batched bits: 20 requests with 20 different
+    # parameter sets submitted at once (set ids up to 19, so slices switch between the two
+    # as requests end), then a 40-region mesh beside a small one.
+    from test_gpu_core import _layered_grid
+
+    _, mid, p, s, seed = small_meshes["sphere"]
+    sets = []
+    for i in range(20):
+        q = type(p).from_buffer_copy(p)
+        q.regions[1].g = 0.80 + 0.005 * i
+        q.regions[2].mus = p.regions[2].mus * (1.0 + 0.02 * i)
+        sets.append(q)
+    refs = [ctx.run_bscan(mid, q, s, seed + i)[0] for i, q in enumerate(sets)]
+    xyz, tets = _layered_grid(n=(8, 8, 40))
+    mesh40 = O.Mesh.build(xyz, tets)
+    mid40 = ctx.upload(mesh40)
+    regs = [configs._reg(1.33 + 0.004 * (i % 7), 0.05, 8.0, 0.9) for i in range(40)]
+    p40 = O.make_params(regs, num_bins=128, depth=0.06, **configs.BIAS)
+    root, z = mesh40.find_root(0.0, 0.0, 0.0)
+    s40 = [O.ascan(0.0, 0.0, z, root, 900), O.ascan(0.02, 0.0, z, mesh40.find_root(0.02, 0.0, 0.0)[0], 700, 1)]
+    ref40 = ctx.run_bscan(mid40, p40, s40, 5)[0]
+    _, mids, ps, ss, seeds = small_meshes["slab"]
+    ref_slab = ctx.run_bscan(mids, ps, ss, seeds)[0]
+    srv = O.Server(ctx, slice_photons=4096)
+    try:
+        ids = [srv.submit(mid, q, s, seed + i) for i, q in enumerate(sets)]
+        for i, rid in enumerate(ids):
+            info, out = srv.wait(rid, 120)
+            assert info.status == 0 and np.array_equal(out, refs[i]), i
+        r40, rslab = srv.submit(mid40, p40, s40, 5), srv.submit(mids, ps, ss, seeds)
+        info, out = srv.wait(r40, 120)
+        assert info.status == 0 and np.array_equal(out, ref40)
+        info, out = srv.wait(rslab, 120)
+        assert info.status == 0 and np.array_equal(out, ref_slab)
+    finally:
+        srv.close()
+
+
 def test_serving_failure_is_retried_deterministically(ctx, small_meshes):
     _, mid, p, s, seed = small_meshes["sphere"]
     ref = ctx.run_bscan(mid, p, s, seed)[0]
