@@ -170,8 +170,8 @@ def main():
               file=sys.stderr, flush=True)
     # Warmup on a small slice of the same B-scan (code object load, buffer allocation), then W
     # full steps. The slice runs through the logged variant of the kernel, a different kernel
-    # symbol, so a profiler's per-kernel average of transport_kernel<false, false> covers the
-    # timed launches only.
+    # symbol, so a profiler's per-kernel average of the unlogged single-set transport_kernel
+    # covers the timed launches only.
     small = [O.ascan(s.x, s.y, s.z, s.root_tet, min(s.n_photons, 500), s.stream_salt) for s in scans]
     ctx.run_bscan(mid, params, small, seed, opts=opts, log=True)
     for i in range(a.warmup):
