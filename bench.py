@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-window", type=float, default=0.1, help="seconds of arrivals in the served stream")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-same", action="store_true",
+                    help="also submit the B-scan as one request to the server and compare its tallies")
     ap.add_argument("--slab128-cells", type=int, default=128)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a plain copy kernel of this size first (byte-counter calibration)")
@@ -257,6 +259,24 @@ def main():
         print(f"[bench] cpu oracle: {n} photons in {cw:.1f}s; same photons on the GPU: summed reflectance "
               f"{gpu_sum:.6e} vs oracle {oracle_sum:.6e}", file=sys.stderr, flush=True)
 
+    served_same = None
+    if a.serve_same:
+        # The same B-scan as one request to the request server (the multi-set kernel in
+        # wall-clock slices), after the timed region: its rate beside the batched launch and
+        # its tallies against the batched launch's, bit for bit.
+        srv = O.Server(ctx)
+        try:
+            ts = time.perf_counter()
+            info, out = srv.wait(srv.submit(mid, params, scans, seed), -1.0)
+            sw = time.perf_counter() - ts
+            sst = srv.stats()
+        finally:
+            srv.close()
+        served_same = {"photons_per_s": total_photons / sw, "wall_s": sw, "slices": sst["slices"],
+                       "status": info.status, "equal_to_batched": bool(np.array_equal(out, batched_tally))}
+        print(f"[bench] B-scan as one served request: {total_photons / sw:.3e} photons/s, "
+              f"equal to batched: {served_same['equal_to_batched']}", file=sys.stderr, flush=True)
+
     serving = None
     if not a.no_serve:
         serving = run_serving(O, configs, ctx, a, lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True))
@@ -305,6 +325,7 @@ def main():
         "gather_roofline": gather,
         "cpu_baseline": cpu,
         "serving": serving,
+        "served_same_bscan": served_same,
     }
     print(json.dumps(line), flush=True)
 
