@@ -822,12 +822,7 @@ hipError_t launch_transport(const KArgs &a, int grid, int block, bool multi, boo
 }
 
 int transport_blocks_per_cu(int block, bool multi, bool log) {
-    int n = 0;
-    hipError_t e;
-    if (multi) {
-        (void)e;
-        return log ? occupancy_multi<true>(block) : occupancy_multi<false>(block);
-    }
+    if (multi) return log ? occupancy_multi<true>(block) : occupancy_multi<false>(block);
     return log ? occupancy_single<true>(block) : occupancy_single<false>(block);
 }
 
