@@ -29,12 +29,12 @@
 // MWC draws and stream seeding against big-integer arithmetic; Fresnel against the textbook
 // formula; the HG and biased samplers as inverse CDFs; equations (8) and (9) through their
 // sphere integrals; Spin's mean direction; HitBoundary against a barycentric brute force;
-// the detector tally's window, counters and class rule on enumerated edge cases; the whole
-// loop against a glass-slab known answer. Parity unpinned (the reference ships no
-// golden outputs and its kernel cannot run here): SpinBias's flag transitions
-// (src/octmps_kernel.cuh:918-968), the detector filter rule (:674-681) and the
-// continuation-photon restore (:1376-1387) are checked only through those composed
-// identities and the known answer, not against reference output.
+// the detector tally's window, counters and class rule on enumerated edge cases; SpinBias's
+// branch choice (src/octmps_kernel.cuh:918-968), the detector filter rule (:667-681) and the
+// roulette with the continuation-photon restore (:1376-1391) on enumerated states, one per
+// side of each condition (tests/test_cpu_oracle_branches.py); the whole loop against a
+// glass-slab known answer. The reference ships no golden outputs and its kernel cannot run
+// here, so none of these is compared with reference output.
 #include <atomic>
 #include <cfloat>
 #include <cmath>
@@ -335,6 +335,20 @@ void tally_detection(const Sim &s, const Photon &p, double rF, Tally &T, PhotonL
 }
 
 // FastReflectTransmit, src/octmps_kernel.cuh:590-805 (PARTIAL_REFLECTION = 1).
+// The spatial filter of a photon leaving through the top surface (uz < 0),
+// src/octmps_kernel.cuh:667-681: inside the collecting radius (strict) and cone (strict),
+// and then TypeBias 0 or 3, or the TypeBias-37 photon was biased backwards, or (TypeBias
+// other than 37) its first bias happened at its maximum depth, or it met a backward
+// specular reflection.
+bool detect_filter(const Sim &s, const Photon &p, double px, double py) {
+    if (!(sqrt((p.x - px) * (p.x - px) + (p.y - py) * (p.y - py)) < s.MaxCollectingRadius &&
+          acos(-p.uz) < s.MaxCollectingAngleDeg * kPi / 180))
+        return false;
+    if (s.TypeBias == 0 || s.TypeBias == 3) return true;
+    return (p.FstBackReflectionFlag && s.TypeBias == 37) || (p.LocationFstBias == p.MaxDepth && s.TypeBias != 37) ||
+           p.NumBackwardsSpecularReflections > 0;
+}
+
 void reflect_transmit(const Sim &s, Photon &p, Rng &rr, double px, double py, double pz, Tally &T,
                       PhotonLog *log) {
     uint32_t t = p.rootIdx;
@@ -354,16 +368,7 @@ void reflect_transmit(const Sim &s, Photon &p, Rng &rr, double px, double py, do
     if (nidx == -1 && rF < 1.0) {
         p.ux = uxr; p.uy = uyr; p.uz = uzr;
         if (p.uz < 0) {
-            bool flag = false;
-            if (sqrt((p.x - px) * (p.x - px) + (p.y - py) * (p.y - py)) < s.MaxCollectingRadius &&
-                acos(-p.uz) < s.MaxCollectingAngleDeg * kPi / 180) {
-                if (s.TypeBias == 0 || s.TypeBias == 3) flag = true;
-                if ((p.FstBackReflectionFlag && s.TypeBias == 37) ||
-                    (p.LocationFstBias == p.MaxDepth && s.TypeBias != 37) ||
-                    p.NumBackwardsSpecularReflections > 0)
-                    flag = true;
-            }
-            if (flag) tally_detection(s, p, rF, T, log);
+            if (detect_filter(s, p, px, py)) tally_detection(s, p, rF, T, log);
         } else {
             p.NumBackwardsSpecularReflections++;
         }
@@ -539,6 +544,31 @@ void spin_bias(const Sim &s, double g, Photon &p, Photon &cont, Rng &r, double p
 enum { EV_LAUNCH = 0, EV_SCATTER = 1, EV_TRANSMIT = 2, EV_REFLECT = 3, EV_BOUNDARY = 4, EV_SURVIVE = 5,
        EV_RESTORE = 6, EV_DEATH = 7, EV_KILLED = 8 };
 
+// Roulette of a photon below the weight threshold, src/octmps_kernel.cuh:1376-1391: it
+// survives with probability CHANCE (weight / CHANCE; a zero weight never survives);
+// otherwise, unless TypeBias is 3, a photon that was biased backwards is replaced by its continuation
+// photon (the copy taken before its first bias; LikelihoodRatioAfterFstBias is not part of
+// the copy), spun with HG and given LR = 1 - LR_afterFstBias (1 when that is >= 1), and
+// any other photon dies.
+enum { kUntouched = 0, kSurvived = 1, kRestored = 2, kDied = 3 };
+int roulette(const Sim &s, Photon &p, const Photon &cont, Rng rng[3]) {
+    if (!(p.w < kWeight)) return kUntouched;
+    double rnd = rand_co(rng[0]);
+    if (p.w != 0.0 && rnd < kChance) {
+        p.w *= div_rd(1.0, kChance);
+        return kSurvived;
+    }
+    if (p.FstBackReflectionFlag && s.TypeBias != 3) {
+        double lrt = p.LikelihoodRatioAfterFstBias;
+        copy_photon(cont, p);
+        spin(s.reg[region_of(s, p.rootIdx)].g, p, rng[2]);
+        if (lrt < 1) p.LikelihoodRatio = 1 - lrt;
+        else p.LikelihoodRatio = 1;
+        return kRestored;
+    }
+    return kDied;
+}
+
 struct TraceSink {
     double *out;
     int cap, n;
@@ -590,22 +620,12 @@ void run_photon(const Sim &s, Rng rng[3], double px, double py, double pz, uint6
             }
             if (tr) tr->add(EV_SCATTER, p, bi.flag, bi.factor);
         }
-        if (p.w < kWeight) {
-            double rnd = rand_co(rng[0]);
-            if (p.w != 0.0 && rnd < kChance) {
-                p.w *= div_rd(1.0, kChance);
-                if (tr) tr->add(EV_SURVIVE, p, 0, 1.0);
-            } else if (p.FstBackReflectionFlag && s.TypeBias != 3) {
-                double lrt = p.LikelihoodRatioAfterFstBias;
-                copy_photon(cont, p);
-                spin(s.reg[region_of(s, p.rootIdx)].g, p, rng[2]);
-                if (lrt < 1) p.LikelihoodRatio = 1 - lrt;
-                else p.LikelihoodRatio = 1;
-                if (tr) tr->add(EV_RESTORE, p, 0, 1.0);
-            } else {
-                if (tr) tr->add(EV_DEATH, p, 0, 1.0);
-                return;
-            }
+        const int fate = roulette(s, p, cont, rng);
+        if (fate == kSurvived && tr) tr->add(EV_SURVIVE, p, 0, 1.0);
+        if (fate == kRestored && tr) tr->add(EV_RESTORE, p, 0, 1.0);
+        if (fate == kDied) {
+            if (tr) tr->add(EV_DEATH, p, 0, 1.0);
+            return;
         }
     }
 }
@@ -852,6 +872,57 @@ void oracle_spin_bias_batch(const octmps_params *prm, const double *state, doubl
         double *o = out + 8 * (int64_t)i;
         o[0] = p.ux; o[1] = p.uy; o[2] = p.uz; o[3] = bi.factor; o[4] = bi.flag; o[5] = bi.selected;
         o[6] = p.LikelihoodRatio; o[7] = p.FstBackReflectionFlag;
+    }
+}
+
+// Detector filter on one exit state: st[7] = x, y, uz, FstBack, LocationFstBias, MaxDepth,
+// NumBackSpec. Returns 1 when the photon would be tallied.
+int oracle_detect_filter(const octmps_params *prm, const double *st, double px, double py) {
+    Sim s;
+    make_sim(prm, s);
+    Photon p;
+    std::memset(&p, 0, sizeof p);
+    p.x = st[0]; p.y = st[1]; p.uz = st[2];
+    p.FstBackReflectionFlag = (uint32_t)st[3];
+    p.LocationFstBias = st[4];
+    p.MaxDepth = st[5];
+    p.NumBackwardsSpecularReflections = (uint32_t)st[6];
+    return detect_filter(s, p, px, py) ? 1 : 0;
+}
+
+// n independent roulettes of copies of one photon and its continuation photon, in a
+// one-tet mesh of region 1 with HG parameter g; streams per call from (seed, call index).
+// ph[10] and ct[10] = x,y,z,ux,uy,uz,w,LR,LR_afterFstBias,FstBack.
+// out[n][10] = fate (0 untouched, 1 survived, 2 restored, 3 died), x,y,z,ux,uy,uz,w,LR,FstBack.
+void oracle_roulette_batch(const octmps_params *prm, const double *ph, const double *ct, double g, uint64_t seed,
+                           const uint32_t *mult, uint32_t K, int n, double *out) {
+    Sim s;
+    make_sim(prm, s);
+    int32_t link[4] = {-1, -1, -1, -1}, reg[1] = {1};
+    double planes[16] = {0};
+    s.planes = planes;
+    s.links = link;
+    s.regions = reg;
+    s.reg[1].g = g;
+    auto load = [](const double *v, Photon &q) {
+        std::memset(&q, 0, sizeof q);
+        q.x = v[0]; q.y = v[1]; q.z = v[2];
+        q.ux = v[3]; q.uy = v[4]; q.uz = v[5];
+        q.w = v[6]; q.LikelihoodRatio = v[7]; q.LikelihoodRatioAfterFstBias = v[8];
+        q.FstBackReflectionFlag = (uint32_t)v[9];
+        q.LocationFstBias = -1.0;
+        q.rootIdx = 0;
+    };
+    for (int i = 0; i < n; ++i) {
+        Photon p, cont;
+        load(ph, p);
+        load(ct, cont);
+        Rng r[3];
+        streams_for(seed, 0, (uint64_t)i, mult, K, r);
+        const int fate = roulette(s, p, cont, r);
+        double *o = out + 10 * (int64_t)i;
+        o[0] = fate; o[1] = p.x; o[2] = p.y; o[3] = p.z; o[4] = p.ux; o[5] = p.uy; o[6] = p.uz;
+        o[7] = p.w; o[8] = p.LikelihoodRatio; o[9] = p.FstBackReflectionFlag;
     }
 }
 

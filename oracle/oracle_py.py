@@ -49,6 +49,11 @@ def lib():
         L.oracle_tally_detection.restype = None
         L.oracle_tally_detection.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
                                              C.c_double, dp, up, up]
+        L.oracle_detect_filter.restype = C.c_int
+        L.oracle_detect_filter.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
+        L.oracle_roulette_batch.restype = None
+        L.oracle_roulette_batch.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_uint64, u32p, C.c_uint32, C.c_int,
+                                            dp]
         _lib = L
     return _lib
 
@@ -161,6 +166,21 @@ def tally_detection(params, op, maxz, w, r_fresnel, lr):
     lib().oracle_tally_detection(C.addressof(params), op, maxz, w, r_fresnel, lr, _dp(sums), _up(counts),
                                  _up(counters))
     return sums.reshape(2, B), counts.reshape(2, B), counters
+
+
+def detect_filter(params, x, y, uz, fst_back, loc_fst, max_depth, nbs, probe=(0.0, 0.0)):
+    st = np.array([x, y, uz, fst_back, loc_fst, max_depth, nbs], np.float64)
+    return lib().oracle_detect_filter(C.addressof(params), _dp(st), probe[0], probe[1])
+
+
+def roulette_batch(params, photon, cont, g, seed, table, n):
+    """photon, cont: x,y,z,ux,uy,uz,w,LR,LR_afterFstBias,FstBack. Rows: fate, x,y,z,ux,uy,uz,w,LR,FstBack."""
+    ph = np.ascontiguousarray(photon, np.float64)
+    ct = np.ascontiguousarray(cont, np.float64)
+    out = np.zeros((n, 10), np.float64)
+    lib().oracle_roulette_batch(C.addressof(params), _dp(ph), _dp(ct), g, seed, _u32p(table.mult), table.k, n,
+                                _dp(out))
+    return out
 
 
 def spin_bias_batch(params, state, probe, seed, table, n):
