@@ -141,6 +141,37 @@ def test_many_regions_escaped_links_parity_paired(ctx, table):
     assert np.array_equal(ta, tb[0])
 
 
+def test_bump_slope_parity_paired(ctx, table):
+    # A probe on the bump's flank (x = 0.35 mm, surface tilted ~27 deg): the root tet comes
+    # from projecting the probe onto the surface, photons enter and leave through tilted
+    # boundary faces, and Fresnel and the detector's cone see non-axis-aligned planes. A
+    # wide detector (0.5 mm, 30 deg) collects light leaving the slope. Its likelihood
+    # ratios are heavy-tailed: ten oracle runs of 16,000 photons with different seeds give
+    # total R with a 33 % relative spread, so a fixed 0.5 % bound on total R is far below
+    # the noise of any affordable run. Total R is tested instead through the paired
+    # per-photon contributions (GPU minus oracle, photon by photon), whose mean must lie
+    # within 5 standard errors of zero; per-bin z scores as in the other cases.
+    n = 16000
+    mesh, params, _, seed = configs.bump(cells=(48, 48, 24), n_scans=1, photons=1, bins=256)
+    params.max_collecting_radius = 0.05
+    params.max_collecting_angle_deg = 30.0
+    x = 0.35 * configs.MM
+    root, z = mesh.find_root(x, 0.0, 0.0, project=True)
+    assert z < -0.015  # on the flank, well above z = 0
+    scan = O.ascan(x, 0.0, z, root, n)
+    gpu, st, plog, ref = _run_pair(ctx, table, mesh, params, scan, seed, threads=16)
+    assert st.photons == n and st.killed == 0 and ref["killed"] == 0
+    assert ref["num_filtered"] > 1000
+    same_steps = np.mean(plog["steps"].astype(np.float64) == ref["log"][:, 0])
+    c = compare(gpu, ref, n)
+    d = plog["contrib"].astype(np.float64) - ref["log"][:, 2]
+    z_total = d.sum() / (d.std(ddof=1) * np.sqrt(n))
+    print("bump slope", c, "same steps", same_steps, "paired total z", z_total)
+    assert same_steps > 0.5
+    assert c["frac_3sigma"] <= 0.01 and c["max_abs_z"] < 5
+    assert abs(z_total) < 5 and c["R_rel"] < 0.05
+
+
 @pytest.mark.parametrize("type_bias", [0, 3])
 def test_unbiased_and_no_spin_parity_paired(ctx, table, type_bias):
     # TypeBias 0 scatters with Spin (Henyey-Greenstein only) and accepts every photon that
