@@ -295,6 +295,14 @@ def main():
                                  "2 MiB set, 6 waves per SIMD"}
         print(f"[bench] gather ceiling {peak:.3e} lane-loads/s; kernel {got:.3e} ({got / peak:.2f})",
               file=sys.stderr, flush=True)
+    # Measured HBM copy rate on this GPU, beside the nominal peak the roofline uses: 4 GiB
+    # read + 4 GiB written per copy (16x the Infinity Cache), after the timed region.
+    hbm_copy = None
+    if a.workload == "slab4":
+        nb = 4 << 30
+        ms = ctx.probe_copy(nb, 5)
+        hbm_copy = 2 * nb / (ms * 1e-3) / 1e9
+        print(f"[bench] HBM copy: {hbm_copy:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
     ctx.close()
     line = {
         "metric": "photons/s per B-scan, batched vs plain kernel; p50/p99 A-scan request latency",
@@ -321,6 +329,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (steps_per_launch * TRAFFIC_B_PER_STEP) if TRAFFIC_B_PER_STEP else None,
                      "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step",
+                     "peak_kind": "nominal MI355X HBM3E", "measured_copy_GB_s": hbm_copy,
                      "layout_bytes_per_step": LAYOUT_BYTES_PER_STEP},
         "gather_roofline": gather,
         "cpu_baseline": cpu,

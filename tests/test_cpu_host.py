@@ -144,3 +144,24 @@ def test_safeprimes_file_override(tmp_path):
         O.check(O.lib.octmps_set_stream_triplets(4096), "triplets")
     x, a = O.photon_streams(1, 0, 9)
     assert int(a[0]) == int(O.safeprimes(3 + 3 * 10)[3 + 3 * 9])
+
+
+def test_benchmark_workloads_match_committed_traces():
+    # The ellipsoid B-scan's per-position budgets and the served request traces that the
+    # benchmark runs, against the committed fixture (tests/golden/make_workload_traces.py):
+    # a change of generator or of numpy's random streams must not silently change them.
+    import json
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "workload_traces.json")) as f:
+        d = json.load(f)
+    b = configs.ellipsoid_budgets()
+    assert [int(x) for x in b] == d["ellipsoid_budgets_seed7"]
+    assert len(b) == 512 and b.min() >= 1e5 and b.max() <= 1e7
+    # log-uniform on [1e5, 1e7]: mean (1e7 - 1e5) / ln 100 = 2.15e6, log-mean at 1e6
+    assert abs(b.mean() / 2.15e6 - 1) < 0.1 and abs(np.log(b).mean() - np.log(1e6)) < 0.25
+    for s in d["served_seed11"]:
+        got = configs.served_stream(s["rate"], s["window"])
+        assert len(got) == len(s["requests"])
+        for (t, m, n, bud), r in zip(got, s["requests"]):
+            assert t == r["t"] and m == r["mesh"] and n == r["n_scans"] and [int(x) for x in bud] == r["budgets"]
+            assert 0 <= m < 4 and 1 <= n <= 256 and len(bud) == n
