@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-window", type=float, default=0.1, help="seconds of arrivals in the served stream")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-policy", type=int, default=0, choices=(0, 1),
+                    help="request server hand-out order: 0 least remaining photons first, 1 equal shares")
     ap.add_argument("--serve-same", action="store_true",
                     help="also submit the B-scan as one request to the server and compare its tallies")
     ap.add_argument("--slab128-cells", type=int, default=128)
@@ -86,7 +88,7 @@ def run_serving(O, configs, ctx, a, log):
     meshes.append((m, ctx.upload(m), p, O.ascan(0.0, 0.0, z, root, 1)))
     log(f"served meshes resident ({sum(x[0].num_tets for x in meshes)} tets) in {time.time() - t0:.1f}s")
     trace = configs.served_stream(a.serve_rate, a.serve_window, seed=11, n_meshes=len(meshes))
-    srv = O.Server(ctx)
+    srv = O.Server(ctx, policy=a.serve_policy)
     ids = []
     photons = 0
     start = time.perf_counter()
@@ -120,6 +122,7 @@ def run_serving(O, configs, ctx, a, log):
     lat = np.array(lat)
     span_s = (last - first) * 1e-3
     return {"requests": len(ids), "rate_req_s": a.serve_rate, "arrival_window_s": a.serve_window,
+            "policy": ("least remaining photons first, aged", "equal shares")[a.serve_policy],
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "idle_small_request_ms": float(np.median(floor)),
             "photons": photons, "photons_per_s": photons / span_s, "span_s": span_s,

@@ -296,6 +296,11 @@ typedef struct {
     int32_t fail_request_index;  /* test hook: submit index to fail (-1 = none) */
     uint64_t hbm_budget_bytes;   /* admission budget for tallies+state (0 = 1 GiB) */
     uint64_t slice_photons;      /* photon ids handed out per slice (0 = auto) */
+    int32_t policy;              /* order of photon hand-out among requests in flight:
+                                    0 = least remaining photons first, aged by waiting time
+                                    (halved after 10 s, a third after 20 s, ...);
+                                    1 = equal shares per A-scan. Results are the same bits. */
+    int32_t reserved;            /* must be 0 */
 } octmps_serve_opts;
 
 typedef struct {

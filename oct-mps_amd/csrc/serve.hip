@@ -29,6 +29,10 @@ namespace octmps {
 namespace {
 
 constexpr int kMaxSets = 256;
+// Hand-out policies (octmps_serve_opts.policy).
+constexpr int kLeastRemainingFirst = 0, kEqualShares = 1;
+// Waiting time over which a request's remaining photons count half as much when ordering.
+constexpr double kAgingMs = 10000.0;
 
 struct ScanState {
     int slot = -1;
@@ -88,6 +92,7 @@ struct octmps_server {
     std::vector<Range> free_tally;
     uint64_t tally_cap_words = 0;
     bool have_saved = false;
+    uint64_t last_claims = 0;  // photon ids claimed by the last slice
     // stats
     uint64_t st_photons = 0, st_steps = 0, st_slices = 0;
     double st_kernel_ms = 0;
@@ -137,7 +142,9 @@ void release(octmps_server *s, Req *r) {
     for (ScanState &x : r->ss) x.slot = -1;
 }
 
-// Uploads a request's parameter set and scan slots and zeroes its tallies and counters.
+// Uploads a request's parameter set and scan slots and zeroes its tallies and per-slot done
+// counters. The hand-out counters (claimed, limit) belong to the caller, under the lock: a
+// cancel may already have lowered `limit`.
 octmps_status stage(octmps_server *s, Req *r) {
     octmps_ctx *ctx = s->ctx;
     const MeshEntry &me = ctx->meshes[r->mesh_id];
@@ -161,10 +168,6 @@ octmps_status stage(octmps_server *s, Req *r) {
     }
     if ((e = launch_fill_u64((uint64_t *)s->tallies.p + r->tally_base, r->words * r->scans.size(), 0, s->stream)))
         return hip_fail(e, "serve stage");
-    for (ScanState &x : r->ss) {
-        x.claimed = 0;
-        x.limit = x.n;
-    }
     return OCTMPS_OK;
 }
 
@@ -236,31 +239,70 @@ void scheduler(octmps_server *s) {
         // Every live photon belongs to an active request (a request ends only once all
         // photons it handed out are done), so the sets of the active requests are the
         // only ones the slice can touch.
+        // `from` and `rem`: first unclaimed photon and photons left of each open A-scan,
+        // read under the lock (a cancel lowers `limit` from another thread).
         bool sets_in_lds = true;
+        std::vector<uint64_t> from, rem;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (Req *r : s->active) {
                 if (r->set >= kLdsSets || ctx->meshes[r->mesh_id].max_region >= kLdsSetRegions) sets_in_lds = false;
                 for (size_t i = 0; i < r->ss.size(); ++i)
-                    if (r->ss[i].claimed < r->ss[i].limit) open.push_back({r, (int)i});
+                    if (r->ss[i].claimed < r->ss[i].limit) {
+                        open.push_back({r, (int)i});
+                        from.push_back(r->ss[i].claimed);
+                        rem.push_back(r->ss[i].limit - r->ss[i].claimed);
+                    }
             }
         }
         uint64_t budget = s->o.slice_photons ? s->o.slice_photons : s->lanes * 4;
         uint64_t total = 0;
-        if (!open.empty()) {
-            uint64_t chunk = std::max<uint64_t>(256, budget / open.size());
-            std::vector<uint64_t> cur(open.size());
-            for (size_t k = 0; k < open.size(); ++k) cur[k] = open[k].first->ss[open[k].second].claimed;
+        if (!open.empty() && s->o.policy == kLeastRemainingFirst) {
+            // Photon ids are claimed in segment order, so the order of the segments is the
+            // order of service. Requests go by unclaimed photons divided by (1 + wait / 10 s),
+            // which approximates shortest-remaining-first (least mean latency for a burst)
+            // and lets a large request that has waited long enough overtake newer small
+            // ones, so none starves; each request's A-scans follow in order.
+            const double now = now_ms() - s->t0;
+            std::vector<std::pair<double, size_t>> key(open.size());
+            {
+                std::map<Req *, uint64_t> left;
+                for (size_t k = 0; k < open.size(); ++k) left[open[k].first] += rem[k];
+                for (size_t k = 0; k < open.size(); ++k) {
+                    Req *r = open[k].first;
+                    key[k] = {(double)left[r] / (1.0 + std::max(0.0, now - r->t_submit) / kAgingMs), k};
+                }
+            }
+            std::stable_sort(key.begin(), key.end(), [&](const std::pair<double, size_t> &a,
+                                                          const std::pair<double, size_t> &b) {
+                if (a.first != b.first) return a.first < b.first;
+                return open[a.second].first->id < open[b.second].first->id;
+            });
+            for (size_t j = 0; j < key.size() && total < budget; ++j) {
+                const size_t k = key[j].second;
+                const uint64_t len = std::min<uint64_t>(rem[k], budget - total);
+                seg_start.push_back(total);
+                seg_poff.push_back(from[k]);
+                seg_slot.push_back((uint32_t)open[k].first->ss[open[k].second].slot);
+                seg_owner.push_back(open[k]);
+                total += len;
+            }
+        } else if (!open.empty()) {
+            // Equal shares: round-robin chunks sized so that one round covers about as many
+            // ids as the last slice claimed, so every open A-scan is reached in each slice.
+            const uint64_t round = std::max<uint64_t>(s->last_claims, s->lanes / 8);
+            uint64_t chunk = std::max<uint64_t>(256, round / open.size());
+            std::vector<uint64_t> cur = from;
             bool any = true;
             while (any && total < budget) {
                 any = false;
                 for (size_t k = 0; k < open.size() && total < budget; ++k) {
-                    ScanState &x = open[k].first->ss[open[k].second];
-                    if (cur[k] >= x.limit) continue;
-                    uint64_t len = std::min<uint64_t>(chunk, x.limit - cur[k]);
+                    const uint64_t lim = from[k] + rem[k];
+                    if (cur[k] >= lim) continue;
+                    uint64_t len = std::min<uint64_t>(chunk, lim - cur[k]);
                     seg_start.push_back(total);
                     seg_poff.push_back(cur[k]);
-                    seg_slot.push_back((uint32_t)x.slot);
+                    seg_slot.push_back((uint32_t)open[k].first->ss[open[k].second].slot);
                     seg_owner.push_back(open[k]);
                     cur[k] += len;
                     total += len;
@@ -340,18 +382,20 @@ void scheduler(octmps_server *s) {
         s->have_saved = true;
         float kms = 0;
         (void)hipEventElapsedTime(&kms, s->e0, s->e1);
+        // Counters, claims and completions change under the lock: stats and cancel read them.
+        std::lock_guard<std::mutex> lk(s->mu);
         s->st_kernel_ms += kms;
         s->st_steps += ctrl[CTRL_STEPS];
         s->st_slices++;
         // Ids are claimed in increasing order: every segment below `claimed` was handed out.
         uint64_t claimed = std::min<uint64_t>(ctrl[CTRL_NEXT_ID], total);
+        s->last_claims = claimed;
         for (size_t k = 0; k < seg_owner.size(); ++k) {
             uint64_t a = seg_start[k], b = seg_start[k + 1];
             if (claimed <= a) break;
             uint64_t used = std::min(claimed, b) - a;
             seg_owner[k].first->ss[seg_owner[k].second].claimed += used;
         }
-        std::lock_guard<std::mutex> lk(s->mu);
         std::vector<Req *> act = s->active;
         for (Req *r : act) {
             bool complete = true;
@@ -370,7 +414,12 @@ void scheduler(octmps_server *s) {
                 // Injected device fault on this work item: discard and rerun it from its
                 // deterministic streams in the same slots.
                 r->fail_once = false;
-                stage(s, r);
+                for (ScanState &x : r->ss) {
+                    x.claimed = 0;
+                    x.limit = x.n;
+                }
+                octmps_status st = stage(s, r);
+                if (st != OCTMPS_OK) finish(s, r, st);
                 continue;
             }
             hipError_t ec = hipMemcpy(r->out, (uint64_t *)s->tallies.p + r->tally_base,
@@ -401,12 +450,21 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
         delete s;
         return fail(OCTMPS_E_INVALID, "block size must be 64, 128, 192 or 256");
     }
+    if (s->o.policy != kLeastRemainingFirst && s->o.policy != kEqualShares) {
+        delete s;
+        return fail(OCTMPS_E_INVALID, "policy must be 0 (least remaining first) or 1 (equal shares)");
+    }
+    if (s->o.reserved != 0) {
+        delete s;
+        return fail(OCTMPS_E_INVALID, "reserved option must be 0");
+    }
     if (hipError_t se = hipSetDevice(ctx->device); se != hipSuccess) {
         delete s;
         return hip_fail(se, "set device");
     }
     grid_for(ctx, s->block, 0, true, s->grid);
     s->lanes = (uint64_t)s->grid * s->block;
+    s->last_claims = s->lanes;
     uint64_t budget = s->o.hbm_budget_bytes ? s->o.hbm_budget_bytes : (1ull << 30);
     s->tally_cap_words = budget / 8;
     hipError_t e;

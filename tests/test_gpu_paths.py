@@ -136,6 +136,42 @@ def test_served_sets_beyond_the_lds_cache_equal_batched(ctx, small_meshes):
         srv.close()
 
 
+def test_hand_out_policies_give_the_same_bits_and_order_service(ctx, small_meshes):
+    # A large request (4 A-scans x 4e6 photons, ~3 s) and a small one (1 A-scan x 1e5)
+    # submitted together. Least-remaining-first hands out the small request's photons first,
+    # so it ends after about its slowest photon's lifetime and well before the large one;
+    # equal shares give it a fifth of each slice's photons, so it ends later. Both policies
+    # give the batched bits for both requests.
+    _, mid, p, s, seed = small_meshes["slab"]
+    big = [O.ascan(x.x, x.y, x.z, x.root_tet, 4_000_000, x.stream_salt) for x in s]
+    small = [O.ascan(s[0].x, s[0].y, s[0].z, s[0].root_tet, 100_000, 9)]
+    ref_big, _ = ctx.run_bscan(mid, p, big, seed)
+    ref_small, _ = ctx.run_bscan(mid, p, small, seed)
+    lat = {}
+    for policy in (0, 1):
+        srv = O.Server(ctx, policy=policy)
+        try:
+            rb = srv.submit(mid, p, big, seed)
+            rs = srv.submit(mid, p, small, seed)
+            i_s, out_s = srv.wait(rs, 120)
+            i_b, out_b = srv.wait(rb, 120)
+        finally:
+            srv.close()
+        assert i_s.status == 0 and i_b.status == 0
+        assert np.array_equal(out_s, ref_small) and np.array_equal(out_b, ref_big)
+        lat[policy] = (i_s.t_done_ms - i_s.t_submit_ms, i_b.t_done_ms - i_b.t_submit_ms)
+    print("latency small/large (ms): least remaining first", lat[0], "equal shares", lat[1])
+    assert lat[0][0] < 0.5 * lat[0][1]
+    assert lat[0][0] < lat[1][0]
+
+
+@pytest.mark.parametrize("field,value", [("policy", 2), ("policy", -1), ("reserved", 1), ("block_size", 320)])
+def test_invalid_serve_options_are_rejected(ctx, field, value):
+    with pytest.raises(O.OctmpsError) as e:
+        O.Server(ctx, **{field: value})
+    assert e.value.status == 1
+
+
 def test_serving_failure_is_retried_deterministically(ctx, small_meshes):
     _, mid, p, s, seed = small_meshes["sphere"]
     ref = ctx.run_bscan(mid, p, s, seed)[0]
