@@ -231,7 +231,8 @@ void scheduler(octmps_server *s) {
                 finish(s, r, st);
             }
         }
-        // Segments: round-robin chunks over A-scans that still have photons to hand out.
+        // Segments: the photon-id ranges of the A-scans that still have photons to hand
+        // out, in the order the policy gives them.
         std::vector<uint64_t> seg_start, seg_poff;
         std::vector<uint32_t> seg_slot;
         std::vector<std::pair<Req *, int>> seg_owner;
