@@ -24,15 +24,14 @@
 #include "device_types.hpp"
 #include "host_common.hpp"
 #include "runtime_internal.hpp"
+#include "serve_order.hpp"
 
 namespace octmps {
 namespace {
 
 constexpr int kMaxSets = 256;
-// Hand-out policies (octmps_serve_opts.policy).
+// Hand-out policies (octmps_serve_opts.policy), serve_order.hpp.
 constexpr int kLeastRemainingFirst = 0, kEqualShares = 1;
-// Waiting time over which a request's remaining photons count half as much when ordering.
-constexpr double kAgingMs = 10000.0;
 
 struct ScanState {
     int slot = -1;
@@ -240,10 +239,10 @@ void scheduler(octmps_server *s) {
         // Every live photon belongs to an active request (a request ends only once all
         // photons it handed out are done), so the sets of the active requests are the
         // only ones the slice can touch.
-        // `from` and `rem`: first unclaimed photon and photons left of each open A-scan,
-        // read under the lock (a cancel lowers `limit` from another thread).
+        // The open A-scans' first unclaimed photon and photons left are read under the lock
+        // (a cancel lowers `limit` from another thread).
         bool sets_in_lds = true;
-        std::vector<uint64_t> from, rem;
+        std::vector<OpenScan> os;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (Req *r : s->active) {
@@ -251,65 +250,22 @@ void scheduler(octmps_server *s) {
                 for (size_t i = 0; i < r->ss.size(); ++i)
                     if (r->ss[i].claimed < r->ss[i].limit) {
                         open.push_back({r, (int)i});
-                        from.push_back(r->ss[i].claimed);
-                        rem.push_back(r->ss[i].limit - r->ss[i].claimed);
+                        os.push_back({r->id, r->t_submit, r->ss[i].claimed, r->ss[i].limit - r->ss[i].claimed,
+                                      (uint32_t)r->ss[i].slot});
                     }
             }
         }
-        uint64_t budget = s->o.slice_photons ? s->o.slice_photons : s->lanes * 4;
-        uint64_t total = 0;
-        if (!open.empty() && s->o.policy == kLeastRemainingFirst) {
-            // Photon ids are claimed in segment order, so the order of the segments is the
-            // order of service. Requests go by unclaimed photons divided by (1 + wait / 10 s),
-            // which approximates shortest-remaining-first (least mean latency for a burst)
-            // and lets a large request that has waited long enough overtake newer small
-            // ones, so none starves; each request's A-scans follow in order.
-            const double now = now_ms() - s->t0;
-            std::vector<std::pair<double, size_t>> key(open.size());
-            {
-                std::map<Req *, uint64_t> left;
-                for (size_t k = 0; k < open.size(); ++k) left[open[k].first] += rem[k];
-                for (size_t k = 0; k < open.size(); ++k) {
-                    Req *r = open[k].first;
-                    key[k] = {(double)left[r] / (1.0 + std::max(0.0, now - r->t_submit) / kAgingMs), k};
-                }
-            }
-            std::stable_sort(key.begin(), key.end(), [&](const std::pair<double, size_t> &a,
-                                                          const std::pair<double, size_t> &b) {
-                if (a.first != b.first) return a.first < b.first;
-                return open[a.second].first->id < open[b.second].first->id;
-            });
-            for (size_t j = 0; j < key.size() && total < budget; ++j) {
-                const size_t k = key[j].second;
-                const uint64_t len = std::min<uint64_t>(rem[k], budget - total);
-                seg_start.push_back(total);
-                seg_poff.push_back(from[k]);
-                seg_slot.push_back((uint32_t)open[k].first->ss[open[k].second].slot);
-                seg_owner.push_back(open[k]);
-                total += len;
-            }
-        } else if (!open.empty()) {
-            // Equal shares: round-robin chunks sized so that one round covers about as many
-            // ids as the last slice claimed, so every open A-scan is reached in each slice.
-            const uint64_t round = std::max<uint64_t>(s->last_claims, s->lanes / 8);
-            uint64_t chunk = std::max<uint64_t>(256, round / open.size());
-            std::vector<uint64_t> cur = from;
-            bool any = true;
-            while (any && total < budget) {
-                any = false;
-                for (size_t k = 0; k < open.size() && total < budget; ++k) {
-                    const uint64_t lim = from[k] + rem[k];
-                    if (cur[k] >= lim) continue;
-                    uint64_t len = std::min<uint64_t>(chunk, lim - cur[k]);
-                    seg_start.push_back(total);
-                    seg_poff.push_back(cur[k]);
-                    seg_slot.push_back((uint32_t)open[k].first->ss[open[k].second].slot);
-                    seg_owner.push_back(open[k]);
-                    cur[k] += len;
-                    total += len;
-                    any = true;
-                }
-            }
+        const uint64_t budget = s->o.slice_photons ? s->o.slice_photons : s->lanes * 4;
+        std::vector<Segment> segs;
+        const uint64_t total =
+            s->o.policy == kLeastRemainingFirst
+                ? order_least_remaining(os, now_ms() - s->t0, budget, segs)
+                : order_equal_shares(os, std::max<uint64_t>(s->last_claims, s->lanes / 8), budget, segs);
+        for (const Segment &g : segs) {
+            seg_start.push_back(g.start);
+            seg_poff.push_back(g.poff);
+            seg_slot.push_back(g.slot);
+            seg_owner.push_back(open[g.open]);
         }
         seg_start.push_back(total);
         if (seg_slot.empty()) {  // keep the table non-empty for the binary search
