@@ -141,6 +141,26 @@ def test_many_regions_escaped_links_parity_paired(ctx, table):
     assert np.array_equal(ta, tb[0])
 
 
+def test_finite_next_step_parity_paired(ctx, table):
+    # rndStepSizeInTissue = 0.05 (all other cases use 0, an infinite next step): a photon
+    # biased before is turned towards the probe (F = 1) only when -ln(0.05)/mu_t (0.30 mm in
+    # the sphere) reaches the probe and the probe is inside the cone, else it takes the
+    # mixture (F = 2). In oracle traces of 300 photons, ~1,000 scatterings take F = 1 by
+    # that test and ~600 more take F = 2 than with the infinite step: both sides run.
+    n = 20000
+    mesh, params, scans, seed = configs.sphere(cells=16, photons=n)
+    params.rnd_step_size_in_tissue = 0.05
+    gpu, st, plog, ref = _run_pair(ctx, table, mesh, params, scans[0], seed)
+    assert st.photons == n and st.killed == 0 and ref["killed"] == 0
+    assert ref["num_filtered"] > 100
+    same_steps = np.mean(plog["steps"].astype(np.float64) == ref["log"][:, 0])
+    c = compare(gpu, ref, n)
+    print("finite next step", c, "same steps", same_steps)
+    assert same_steps > 0.5
+    assert c["frac_3sigma"] <= 0.01 and c["max_abs_z"] < 5
+    assert c["R_rel"] < 0.005 and c["Rfilt_rel"] < 0.005
+
+
 def test_bump_slope_parity_paired(ctx, table):
     # A probe on the bump's flank (x = 0.35 mm, surface tilted ~27 deg): the root tet comes
     # from projecting the probe onto the surface, photons enter and leave through tilted
