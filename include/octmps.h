@@ -336,14 +336,17 @@ octmps_status octmps_serve_close(octmps_server *s);
 /* Device unit probes (parity of single device functions against the oracle). */
 /* ------------------------------------------------------------------------- */
 /* n_draws co-draws of each of the 3 streams of photons [photon0, photon0+n_photons):
- * out[(p*3 + s)*n_draws + k]. */
+ * out[(p*3 + s)*n_draws + k]. Device rand_MWC_co (src/octmps_kernel.cuh:10-22) on the
+ * per-photon streams. */
 octmps_status octmps_probe_rng(octmps_ctx *ctx, uint64_t seed, uint64_t salt, uint64_t photon0,
                                int32_t n_photons, int32_t n_draws, float *out);
-/* Fresnel reflectance of the device code for n cases (ni, nt, ca1 = cos incidence). */
+/* Fresnel reflectance of the device code for n cases (ni, nt, ca1 = cos incidence): the
+ * MCML formula of FastReflectTransmit (src/octmps_kernel.cuh:601-646). */
 octmps_status octmps_probe_fresnel(octmps_ctx *ctx, int32_t n, const float *ni, const float *nt,
                                    const float *ca1, float *r_out);
-/* cos(theta) from the HG sampler (kind 0, parameter g) or the biased forward sampler
- * (kind 1, coefficient a) for uniform draws u in (0,1]. */
+/* cos(theta) from the HG sampler (kind 0, parameter g; SpinTheta, src/octmps_kernel.cuh:807-820)
+ * or the biased forward sampler (kind 1, coefficient a; SpinThetaForwardFstBias, :881-896)
+ * for uniform draws u in (0,1]. */
 octmps_status octmps_probe_sampler(octmps_ctx *ctx, int32_t kind, float param, int32_t n,
                                    const float *u, float *cost_out);
 
