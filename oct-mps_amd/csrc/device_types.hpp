@@ -21,8 +21,9 @@
 namespace octmps {
 
 constexpr int kMaxRegions = 100;
-// Parameter sets (and regions of each) a multi-set launch can keep in LDS.
-constexpr int kLdsSets = 16, kLdsSetRegions = 16;
+// Parameter sets (and regions of each) a multi-set launch can keep in LDS, and the number
+// of parameter set ids a request server hands out.
+constexpr int kLdsSets = 16, kLdsSetRegions = 16, kMaxServeSets = 256;
 constexpr uint32_t kLinkBits = 25;                       // pool tet id bits of a packed link
 constexpr uint32_t kLinkMask = (1u << kLinkBits) - 1u;   // also the boundary id
 constexpr int64_t kMaxPoolTets = (int64_t)kLinkMask;     // ids 0 .. 2^25 - 2
@@ -138,7 +139,9 @@ struct KArgs {
     const TetRec *tets;
     uint32_t tets_bytes;        // bytes of resident records (<= (2^25 - 1) * 128 < 2^32)
     uint32_t skip_entry;        // 1: the plane of a tet's entry face is not fetched
-    uint32_t sets_in_lds;       // multi-set launch: every lane's set < kLdsSets, regions < kLdsSetRegions
+    uint32_t sets_in_lds;       // multi-set launch: every lane's set is in the LDS map, regions < kLdsSetRegions
+    const uint8_t *set_slot;    // sets_in_lds: LDS slot of each parameter set id [kMaxServeSets]
+    const int32_t *lds_set_ids; // sets_in_lds: parameter set id held in each LDS slot [kLdsSets]
     const DevRegion *regions;
     const DevSet *sets;
     const DevScan *slots;

@@ -29,7 +29,7 @@
 namespace octmps {
 namespace {
 
-constexpr int kMaxSets = 256;
+constexpr int kMaxSets = kMaxServeSets;
 // Hand-out policies (octmps_serve_opts.policy), serve_order.hpp.
 constexpr int kLeastRemainingFirst = 0, kEqualShares = 1;
 
@@ -83,7 +83,10 @@ struct octmps_server {
     // device state (owned by the scheduler thread)
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    DevBuf regions, sets, slots, seg_start, seg_slot, seg_poff, tallies, ctrl, slot_done, save, cont;
+    DevBuf regions, sets, slots, seg_start, seg_slot, seg_poff, tallies, ctrl, slot_done, save, cont, set_map;
+    // Host copy of set_map: the LDS slot of each set id, then the set id of each LDS slot.
+    uint8_t set_slot_h[kMaxServeSets] = {};
+    int32_t lds_ids_h[kLdsSets] = {};
     int grid = 0, block = 0, q = 0;
     uint64_t lanes = 0;
     int n_slots = 0;
@@ -130,8 +133,8 @@ void release(octmps_server *s, Req *r) {
     for (ScanState &x : r->ss)
         if (x.slot >= 0) s->free_slots.push_back(x.slot);
     if (r->set >= 0) {
-        // Smallest id first, so that up to kLdsSets requests in flight keep their
-        // parameter sets in LDS.
+        // Smallest id first: ids stay dense (the slice maps the sets in flight to LDS
+        // slots whatever their ids).
         s->free_sets.push_back(r->set);
         std::sort(s->free_sets.begin(), s->free_sets.end(), std::greater<int>());
     }
@@ -238,15 +241,21 @@ void scheduler(octmps_server *s) {
         std::vector<std::pair<Req *, int>> open;
         // Every live photon belongs to an active request (a request ends only once all
         // photons it handed out are done), so the sets of the active requests are the
-        // only ones the slice can touch.
+        // only ones the slice can touch: when there are at most kLdsSets of them and their
+        // meshes' regions fit, the slice keeps them in LDS through the set_map.
         // The open A-scans' first unclaimed photon and photons left are read under the lock
         // (a cancel lowers `limit` from another thread).
         bool sets_in_lds = true;
+        int n_lds = 0;
         std::vector<OpenScan> os;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (Req *r : s->active) {
-                if (r->set >= kLdsSets || ctx->meshes[r->mesh_id].max_region >= kLdsSetRegions) sets_in_lds = false;
+                if (ctx->meshes[r->mesh_id].max_region >= kLdsSetRegions || n_lds == kLdsSets) sets_in_lds = false;
+                if (sets_in_lds) {
+                    s->set_slot_h[r->set] = (uint8_t)n_lds;
+                    s->lds_ids_h[n_lds++] = r->set;
+                }
                 for (size_t i = 0; i < r->ss.size(); ++i)
                     if (r->ss[i].claimed < r->ss[i].limit) {
                         open.push_back({r, (int)i});
@@ -273,6 +282,7 @@ void scheduler(octmps_server *s) {
             seg_poff.push_back(0);
             seg_start.insert(seg_start.begin(), 0);
         }
+        for (int j = n_lds; j < kLdsSets; ++j) s->lds_ids_h[j] = s->lds_ids_h[0];  // unused slots: any valid id
         hipError_t e;
         bool ok = (e = s->seg_start.ensure(seg_start.size() * 8)) == hipSuccess &&
                   (e = s->seg_slot.ensure(seg_slot.size() * 4)) == hipSuccess &&
@@ -284,6 +294,11 @@ void scheduler(octmps_server *s) {
                   (e = hipMemcpyAsync(s->seg_poff.p, seg_poff.data(), seg_poff.size() * 8, hipMemcpyHostToDevice,
                                       s->stream)) == hipSuccess &&
                   (e = launch_fill_u64((uint64_t *)s->ctrl.p, CTRL_WORDS, 0, s->stream)) == hipSuccess;
+        if (ok && sets_in_lds)
+            ok = (e = hipMemcpyAsync(s->set_map.p, s->set_slot_h, sizeof s->set_slot_h, hipMemcpyHostToDevice,
+                                     s->stream)) == hipSuccess &&
+                 (e = hipMemcpyAsync((uint8_t *)s->set_map.p + sizeof s->set_slot_h, s->lds_ids_h,
+                                     sizeof s->lds_ids_h, hipMemcpyHostToDevice, s->stream)) == hipSuccess;
         KArgs A;
         std::memset(&A, 0, sizeof A);
         A.tets = ctx->pool_tets;
@@ -314,6 +329,8 @@ void scheduler(octmps_server *s) {
         A.restore = s->have_saved ? 1u : 0u;
         A.exit_on_drain = total > 0 ? 1u : 0u;
         A.sets_in_lds = sets_in_lds ? 1u : 0u;
+        A.set_slot = (const uint8_t *)s->set_map.p;
+        A.lds_set_ids = (const int32_t *)((const uint8_t *)s->set_map.p + sizeof s->set_slot_h);
         uint64_t ctrl[CTRL_WORDS] = {0};
         if (ok) ok = (e = hipEventRecord(s->e0, s->stream)) == hipSuccess &&
                      (e = launch_transport(A, s->grid, s->block, true, false, s->stream)) == hipSuccess &&
@@ -429,6 +446,7 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
         (e = hipEventCreate(&s->e0)) != hipSuccess || (e = hipEventCreate(&s->e1)) != hipSuccess ||
         (e = s->regions.ensure((size_t)kMaxSets * kMaxRegions * sizeof(DevRegion))) != hipSuccess ||
         (e = s->sets.ensure((size_t)kMaxSets * sizeof(DevSet))) != hipSuccess ||
+        (e = s->set_map.ensure(sizeof s->set_slot_h + sizeof s->lds_ids_h)) != hipSuccess ||
         (e = s->slots.ensure((size_t)s->n_slots * sizeof(DevScan))) != hipSuccess ||
         (e = s->slot_done.ensure((size_t)s->n_slots * 8)) != hipSuccess ||
         (e = s->ctrl.ensure(CTRL_WORDS * 8)) != hipSuccess ||
@@ -570,7 +588,7 @@ extern "C" octmps_status octmps_serve_close(octmps_server *s) {
     (void)hipSetDevice(s->ctx->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (DevBuf *b : {&s->regions, &s->sets, &s->slots, &s->seg_start, &s->seg_slot, &s->seg_poff, &s->tallies,
-                      &s->ctrl, &s->slot_done, &s->save, &s->cont})
+                      &s->ctrl, &s->slot_done, &s->save, &s->cont, &s->set_map})
         b->release();
     if (s->e0) (void)hipEventDestroy(s->e0);
     if (s->e1) (void)hipEventDestroy(s->e1);

@@ -214,10 +214,11 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
 // when many records miss the caches (an 805 MB mesh ran 30 % faster than with SKIP, and 50 %
 // faster than with the same offsets computed per plane); with SKIP the smaller gather wins
 // when records come from cache (+2 % on a 101 MB mesh). The launch picks one per mesh.
-// SETS_LDS (MULTI only): the parameter sets 0..kLdsSets-1 and their regions
+// SETS_LDS (MULTI only): up to kLdsSets parameter sets (A.lds_set_ids) and their regions
 // 0..kLdsSetRegions-1 are copied to LDS at launch, and every lane's set must be among them;
-// otherwise each lane reads its set's records from memory, a per-lane gather in the parked
-// phase that made serving 25 % slower than a single-set launch.
+// a lane then carries its set's LDS slot (A.set_slot) in place of the set id. Otherwise
+// each lane reads its set's records from memory, a per-lane gather in the parked phase
+// that made serving 25 % slower than a single-set launch.
 template <bool MULTI, bool LOG, bool SKIP, bool SETS_LDS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) transport_kernel(KArgs A) {
     const uint32_t lane = __lane_id();
@@ -255,8 +256,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
         __syncthreads();
     } else if (SETS_LDS) {
         for (int i = threadIdx.x; i < kLdsSets * kLdsSetRegions; i += blockDim.x)
-            lds_regions[i] = A.regions[(i / kLdsSetRegions) * kMaxRegions + i % kLdsSetRegions];
-        for (int i = threadIdx.x; i < kLdsSets; i += blockDim.x) lds_sets[i] = A.sets[i];
+            lds_regions[i] = A.regions[A.lds_set_ids[i / kLdsSetRegions] * kMaxRegions + i % kLdsSetRegions];
+        for (int i = threadIdx.x; i < kLdsSets; i += blockDim.x) lds_sets[i] = A.sets[A.lds_set_ids[i]];
         __syncthreads();
     }
     // Per-lane LDS stash (blocks have at most 256 lanes): the probe position of the lane's
@@ -288,6 +289,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             aG = s.aG; aR = s.aR; aS = s.aS; sG = s.sG; sR = s.sR; sS = s.sS;
             slot = s.slot; id = s.id; ndet = s.ndet; csum = s.csum;
             set = MULTI ? A.slots[slot].set : 0;
+            if (MULTI && SETS_LDS) set = A.set_slot[set];
             load_region(region);
             const DevScan &sc = A.slots[slot];
             lds_probe[threadIdx.x] = make_float4(sc.px, sc.py, sc.pz, 0.0f);
@@ -356,6 +358,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                         tet = sc.root_tet;
                         region = sc.root_region;
                         set = MULTI ? sc.set : 0;
+                        if (MULTI && SETS_LDS) set = A.set_slot[set];
                         flags = F_ALIVE;
                         steps = 0;
                         id = nid;
