@@ -46,7 +46,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-photons", type=int, default=3000, help="oracle photons per sampled A-scan")
     ap.add_argument("--cpu-scans", type=int, default=8)
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (context defaults: scatter/refill quanta 40/16, block 256)")
+    ap.add_argument("--quanta", default="",
+                    help="scatter,refill quanta to launch with instead of autotuning (e.g. 36,16: the same "
+                         "launch a profiler run can then attribute to one kernel dispatch)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-window", type=float, default=0.1, help="seconds of arrivals in the served stream")
     ap.add_argument("--no-serve", action="store_true")
@@ -168,11 +172,25 @@ def main():
               file=sys.stderr, flush=True)
     opts = O.BatchOpts()
     opts.use_graph = 1
-    if a.autotune:
-        o, rate = ctx.autotune(mid, params, scans, seed, calib=1 << 21)
-        opts.block_size, opts.refill_quantum = o.block_size, o.refill_quantum
-        print(f"[bench] autotune: block {o.block_size} quantum {o.refill_quantum} ({rate:.3e} photons/s)",
-              file=sys.stderr, flush=True)
+    tuned = None
+    if a.quanta:
+        opts.scatter_quantum, opts.refill_quantum = (int(v) for v in a.quanta.split(","))
+        tuned = {"scatter_quantum": opts.scatter_quantum, "refill_quantum": opts.refill_quantum, "source": "--quanta"}
+    elif not a.no_autotune:
+        # The autotuner (coordinate search over launch options on a calibration slice of the
+        # same B-scan: the first 1/16 of every A-scan's photons at the default size), before
+        # the timed region; results are the same bits under any options.
+        calib = max(1, sum(s.n_photons for s in scans) // 16)
+        ts = time.perf_counter()
+        o, rate = ctx.autotune(mid, params, scans, seed, calib=calib)
+        opts.block_size, opts.refill_quantum, opts.scatter_quantum = o.block_size, o.refill_quantum, o.scatter_quantum
+        opts.tally_copies, opts.entry_skip = o.tally_copies, o.entry_skip
+        tuned = {"block_size": o.block_size, "refill_quantum": o.refill_quantum,
+                 "scatter_quantum": o.scatter_quantum, "tally_copies": o.tally_copies, "entry_skip": o.entry_skip,
+                 "calibration_photons": calib, "seconds": time.perf_counter() - ts}
+        print(f"[bench] autotune ({tuned['seconds']:.1f} s): block {o.block_size} scatter/refill quanta "
+              f"{o.scatter_quantum}/{o.refill_quantum} tally copies {o.tally_copies} entry skip {o.entry_skip} "
+              f"({rate:.3e} photons/s on the calibration slice)", file=sys.stderr, flush=True)
     # Warmup on a small slice of the same B-scan (code object load, buffer allocation), then W
     # full steps. The slice runs through the logged variant of the kernel, a different kernel
     # symbol, so a profiler's per-kernel average of the unlogged single-set transport_kernel
@@ -321,7 +339,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc.format(n=len(scans)),
-                   "photons": total_photons, "path": "batched persistent kernel, hipGraph launch"},
+                   "photons": total_photons, "path": "batched persistent kernel, hipGraph launch",
+                   "launch_options": tuned or "context defaults"},
         "batched": {"photons_per_s": value, "kernel_ms": kms, "photon_steps": steps,
                     "steps_per_photon": steps_per_launch / total_photons,
                     "lane_utilisation": sum(steps) / (64.0 * max(1, sum(iters)))},

@@ -276,7 +276,7 @@ octmps_status octmps_run_bscan_logged(octmps_ctx *ctx, int32_t mesh_id, const oc
                                       octmps_run_stats *stats, void *photon_log,
                                       uint64_t log_records);
 /* Autotune scatter quantum, block size, refill quantum, tally privatisation depth and the
- * entry-plane skip on a calibration slice of a B-scan (coordinate search, 16 launches);
+ * entry-plane skip on a calibration slice of a B-scan (coordinate search, 25 launches);
  * writes the winner into *opts (scatter_quantum, block_size, refill_quantum, tally_copies,
  * entry_skip), makes the first four the context's default (entry_skip stays per launch, since
  * its best value depends on the mesh), and returns its photons/s in *rate. */

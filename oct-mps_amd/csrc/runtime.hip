@@ -504,9 +504,10 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     uint64_t per = std::max<uint64_t>(1, calib / (uint64_t)n);
     for (auto &s : cs) s.n_photons = std::min<uint64_t>(s.n_photons, per);
     std::vector<uint64_t> t((size_t)n * tally_words(p->num_depth_bins));
-    // Coordinate search: scatter quantum, then block size, then refill quantum, then tally
-    // privatisation depth, then the entry-plane skip, each with the best values found so far
-    // for the others (16 calibration launches).
+    // Coordinate search: scatter quantum, then block size, then refill quantum, scatter
+    // quantum again (the two quanta interact: fewer parked lanes per scatter phase pair with
+    // more dead lanes per refill), then tally privatisation depth, then the entry-plane
+    // skip, each with the best values found so far for the others (25 calibration launches).
     double best = -1;
     octmps_batch_opts win = *opts;
     win.block_size = 256;
@@ -527,7 +528,8 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
         return OCTMPS_OK;
     };
     octmps_status s;
-    for (int sq : {32, 36, 40, 44}) {
+    const int scatter_quanta[] = {24, 28, 32, 36, 40, 44};
+    for (int sq : scatter_quanta) {
         octmps_batch_opts o = win;
         o.scatter_quantum = sq;
         if ((s = trial(o)) != OCTMPS_OK) return s;
@@ -537,9 +539,14 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
         o.block_size = b;
         if ((s = trial(o)) != OCTMPS_OK) return s;
     }
-    for (int q : {8, 12, 16, 24}) {
+    for (int q : {8, 12, 16, 20, 24}) {
         octmps_batch_opts o = win;
         o.refill_quantum = q;
+        if ((s = trial(o)) != OCTMPS_OK) return s;
+    }
+    for (int sq : scatter_quanta) {
+        octmps_batch_opts o = win;
+        o.scatter_quantum = sq;
         if ((s = trial(o)) != OCTMPS_OK) return s;
     }
     for (int k : {2, 4, 8}) {
