@@ -26,7 +26,7 @@ LAYOUT_BYTES_PER_STEP = 80  # this build's packed tet record: 4 planes x 16 B + 
 # (x2.0, the factor by which that counter under-reads a copy kernel of known size) plus
 # WRITE_SIZE (x1.0), divided by the launch's photon-steps. profiles/transport_slab4_pmc.md
 # holds the counter values and commands.
-TRAFFIC_B_PER_STEP = 26.23
+TRAFFIC_B_PER_STEP = 26.09
 # Measured L1 lookups per photon-step of the batched slab4 launch (rocprofv3 --pmc
 # TCP_TOTAL_CACHE_ACCESSES_sum / photon-steps, profiles/transport_slab4_memory.md): the
 # 16-byte lane-loads of the tet record (5 per visit, 4 after entering through a face) plus
