@@ -3,8 +3,9 @@
 // The reference runs one A-scan at a time from a host loop (src/main.cu:948-1051). Here a
 // scheduler thread keeps meshes resident and runs the transport kernel in slices: before
 // each slice it admits queued requests whose tallies fit the HBM budget and free scan
-// slots, and hands out photon ids round-robin across the A-scans of all admitted requests;
-// during the slice, lanes whose photon dies pick up the next id of any request; at the end
+// slots, and lists the photon ids of all admitted requests in the order of its hand-out
+// policy (least remaining photons first with aging, or equal shares; serve_order.hpp);
+// during the slice, lanes whose photon dies pick up the next listed id; at the end
 // of a slice live photons are saved per lane and resumed by the next slice
 // (SaveThreadState/RestoreThreadState, src/octmps_kernel.cuh:1154-1327). Every photon's
 // streams depend only on (seed, salt, photon index), so a request's tallies are bitwise the
