@@ -127,27 +127,22 @@ PHOTON = [0.01, 0.02, 0.3, *unit((0.1, 0.2, -0.9)), 5e-5, 0.25, 0.3, 1]
 CONT = [0.0, 0.001, 0.04, *unit((0.0, 0.3, 0.95)), 0.61, 0.9, 0.7, 0]
 
 
-def roulette(tb, photon, cont=CONT, n=20000, g=0.8):
-    return OR.roulette_batch(params(type_bias=tb), photon, cont, g, 5, pytest.table, n)
+def roulette(table, tb, photon, cont=CONT, n=20000, g=0.8):
+    return OR.roulette_batch(params(type_bias=tb), photon, cont, g, 5, table, n)
 
 
-@pytest.fixture(autouse=True)
-def _table(table):
-    pytest.table = table
-
-
-def test_roulette_leaves_heavy_photons_alone():
+def test_roulette_leaves_heavy_photons_alone(table):
     ph = list(PHOTON)
     ph[6] = 1e-4                       # WEIGHT itself: not below it
-    out = roulette(37, ph, n=50)
+    out = roulette(table, 37, ph, n=50)
     assert np.all(out[:, 0] == 0) and np.all(out[:, 7] == 1e-4)
 
 
 @pytest.mark.parametrize("lr_after,lr_expect", [(0.3, 0.7), (1.0, 1.0), (1.7, 1.0)])
-def test_roulette_survival_then_restore(lr_after, lr_expect):
+def test_roulette_survival_then_restore(table, lr_after, lr_expect):
     ph = list(PHOTON)
     ph[8] = lr_after
-    out = roulette(37, ph)
+    out = roulette(table, 37, ph)
     fate = out[:, 0]
     assert set(np.unique(fate)) <= {1, 2}
     k = (fate == 1).mean()
@@ -173,8 +168,8 @@ def test_roulette_survival_then_restore(lr_after, lr_expect):
     (37, 0, 0.0, {3}),         # zero weight never survives
     (37, 1, 0.0, {2}),         # ... but is restored when biased backwards
 ])
-def test_roulette_death_and_restore_rules(tb, fst, w, fates):
+def test_roulette_death_and_restore_rules(table, tb, fst, w, fates):
     ph = list(PHOTON)
     ph[9], ph[6] = fst, w
-    out = roulette(tb, ph, n=4000)
+    out = roulette(table, tb, ph, n=4000)
     assert set(np.unique(out[:, 0])) == fates
