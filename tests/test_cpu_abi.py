@@ -73,3 +73,21 @@ def test_struct_layouts_match_the_header(tmp_path):
         cls = getattr(O, py)
         want = ctypes.sizeof(cls) if field == "size" else getattr(cls, field).offset
         assert int(value) == want, (py, field, int(value), want)
+
+
+def test_integration_example_compiles(tmp_path):
+    # The C++ binding example in INTEGRATION.md, wrapped in a main that declares the names it
+    # takes from the reference's host code, must compile against include/octmps.h.
+    import re
+    import subprocess
+
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = re.search(r"```cpp\n(.*?)```", text, re.S).group(1)
+    body = "\n".join(l for l in block.splitlines() if not l.startswith("#include"))
+    src = tmp_path / "example.cpp"
+    src.write_text('#include <cstdio>\n#include <vector>\n#include "octmps.h"\n'
+                   "int main(int argc, char **argv) {\n"
+                   "    const char *filename_opt = argv[1], *filename_bias = argv[2], *filename_mesh = argv[3];\n"
+                   "    uint64_t seed = 1;\n    double x;\n" + body + "\n    return 0;\n}\n")
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src)], check=True)
