@@ -322,3 +322,33 @@ def test_cli_writes_reference_outputs(tmp_path, ctx):
     rows = [r.split(",") for r in (tmp_path / "ClassI_Scatt.out").read_text().splitlines()[:B]]
     got = np.array([float(r[2]) for r in rows])
     assert np.allclose(got, out[0, 0, 0], rtol=1e-5, atol=0)
+
+
+def test_integration_python_example_runs(tmp_path, ctx, monkeypatch):
+    # The Python example of INTEGRATION.md, run as written on a small two-layer case: its
+    # A-scan equals the library's own plain run of the same inputs.
+    import re
+
+    mesh = O.Mesh.generate(O.GEOM_LAYERS, (16, 16, 8), (0.2, 0.2, 0.1), p=(0.02,), n_layers=2)
+    _write_case(tmp_path, mesh)
+    root_dir = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root_dir, "INTEGRATION.md")).read()
+    block = re.search(r"```python\n(.*?)```", text, re.S).group(1)
+    monkeypatch.chdir(tmp_path)
+    ns = {}
+    exec(compile(block, "INTEGRATION.md", "exec"), ns)
+    p = ns["p"]
+    assert ns["stats"].photons == p.number_of_photons == 3000
+    out = ns["mean_se"]  # [filtered][class][mean, SE][bin]
+    assert out.shape == (2, 2, 2, p.num_depth_bins) and np.all(np.isfinite(out[:, :, 0]))
+    # SE is sqrt of (SumSq/N - mean^2)/(N-1) unclamped, as the reference prints it
+    # (src/octmps_io.cpp:385-388, 408-411). The fixed-point SumSq resolves 2^-64, so a
+    # contribution below 2.3e-10 adds nothing to it: a bin fed only by such contributions
+    # gets a negative variance, hence NaN. That happens only where the mean is below 1e-9.
+    se = out[:, :, 1]
+    assert np.all(np.abs(out[:, :, 0][np.isnan(se)]) < 1e-9)
+    assert np.isnan(se).mean() < 0.5
+    ns["ctx"].close()
+    mid = ctx.upload(ns["mesh"])
+    t, _ = ctx.run_ascan(mid, p, O.ascan(0.0, 0.0, ns["z"], ns["root"], 3000), 985456376)
+    assert np.array_equal(t, ns["tally"])
