@@ -174,11 +174,13 @@ octmps_status octmps_photon_streams(uint64_t seed, uint64_t salt, uint64_t photo
 /* Tallies: SimState (src/octmps.h:133-176) as exact integers.                */
 /* One A-scan tally is octmps_tally_words(B) uint64 words:                    */
 /*   [0] NumFilteredPhotons [1] ...ClassI [2] ...ClassII [3] reserved          */
-/*   then for class c in {0: Class I, 1: Class II}, base = 4 + c*6*B:          */
+/*   then for class c in {0: Class I, 1: Class II}, base = 4 + c*7*B:          */
 /*   base+0*B sum_lo, +1*B sum_hi   Sum = sum_hi * 2^-32 + sum_lo * 2^-64      */
-/*   base+2*B sq_lo,  +3*B sq_hi    SumSq, same encoding                       */
-/*   (every contribution c adds floor(c*2^64) split as hi*2^32 + lo, lo < 2^32: */
-/*    exact and carry-free for up to 2^32 contributions per bin)               */
+/*   base+2*B sq_lo,  +3*B sq_hi,  +6*B sq_lo2                                 */
+/*                 SumSq = sq_hi * 2^-32 + sq_lo * 2^-64 + sq_lo2 * 2^-96     */
+/*   (every contribution c adds floor(c*2^64) split as hi*2^32 + lo, lo < 2^32, */
+/*    and c^2 adds floor(c^2*2^96) split the same way over three words: exact  */
+/*    and carry-free for up to 2^32 contributions per bin)                     */
 /*   base+4*B count (NumClass*_PhotonsFilteredInRange)                         */
 /*   base+5*B max   (IEEE-754 binary64 bits of Max)                            */
 /* ------------------------------------------------------------------------- */

@@ -244,10 +244,10 @@ struct Bin {
 };
 
 Bin get_bin(const uint64_t *t, int32_t B, int c, int b) {
-    const uint64_t *base = t + 4 + (uint64_t)c * 6 * B;
+    const uint64_t *base = t + 4 + (uint64_t)c * 7 * B;
     Bin r;
     r.sum = (double)split_to_ld(base[b], base[B + b]);
-    r.sumsq = (double)split_to_ld(base[2 * B + b], base[3 * B + b]);
+    r.sumsq = (double)(split_to_ld(base[2 * B + b], base[3 * B + b]) + (long double)base[6 * B + b] * 0x1p-96L);
     r.count = base[4 * B + b];
     std::memcpy(&r.max, &base[5 * B + b], sizeof(double));
     return r;

@@ -143,7 +143,7 @@ __device__ __forceinline__ void rotate(float ax, float ay, float az, float cost,
 // lo < 2^32. The tally keeps two u64 accumulators, sum(hi) in units of 2^-32 and sum(lo) in
 // units of 2^-64; since each lo is < 2^32 neither word needs a carry, so both adds are
 // non-returning atomics and the total is exact for up to 2^32 adds per bin.
-__device__ __forceinline__ void to_split(float c, uint64_t &lo, uint64_t &hi) {
+__device__ __forceinline__ void to_split(float c, uint32_t &lo, uint64_t &hi) {
     lo = 0;
     hi = 0;
     if (!(c > 0.0f) || !(c < 4.0e9f)) return;
@@ -155,17 +155,48 @@ __device__ __forceinline__ void to_split(float c, uint64_t &lo, uint64_t &hi) {
     int sh = e - 86;  // c * 2^64 = m * 2^(e - 150 + 64), m < 2^24
     if (sh <= -24) return;
     if (sh < 0) {
-        lo = m >> (-sh);
+        lo = (uint32_t)(m >> (-sh));
     } else if (sh < 32) {
         uint64_t v = m << sh;  // < 2^56
-        lo = v & 0xffffffffull;
+        lo = (uint32_t)v;
         hi = v >> 32;
     } else {
         hi = m << (sh - 32);
     }
 }
 
-__device__ __forceinline__ void add_split(uint64_t *lo_p, uint64_t *hi_p, uint64_t lo, uint64_t hi) {
+// The same fixed point 32 bits finer, for SumSq: v = floor(x * 2^96) = w2 * 2^64 + w1 * 2^32 + w0
+// with w0, w1 < 2^32; w2 counts 2^-32 units, w1 2^-64 units (the two words to_split gives) and
+// w0 2^-96 units, so a contribution c down to 3.6e-15 still adds its c^2.
+__device__ __forceinline__ void to_split3(float x, uint32_t &w0, uint32_t &w1, uint64_t &w2) {
+    w0 = 0;
+    w1 = 0;
+    w2 = 0;
+    if (!(x > 0.0f) || !(x < 4.0e9f)) return;
+    uint32_t b = __float_as_uint(x);
+    int e = (int)((b >> 23) & 0xffu);
+    uint64_t m = b & 0x7fffffu;
+    if (e == 0) e = 1;
+    else m |= 0x800000u;
+    int sh = e - 54;  // x * 2^96 = m * 2^(e - 150 + 96), m < 2^24, sh <= 104
+    if (sh <= -24) return;
+    uint64_t lo64, hi64;  // v = hi64 * 2^64 + lo64
+    if (sh < 0) {
+        lo64 = m >> (-sh);
+        hi64 = 0;
+    } else if (sh < 64) {
+        lo64 = m << sh;
+        hi64 = sh > 40 ? m >> (64 - sh) : 0;
+    } else {
+        lo64 = 0;
+        hi64 = m << (sh - 64);
+    }
+    w0 = (uint32_t)lo64;
+    w1 = (uint32_t)(lo64 >> 32);
+    w2 = hi64;
+}
+
+__device__ __forceinline__ void add_split(uint64_t *lo_p, uint64_t *hi_p, uint32_t lo, uint64_t hi) {
     if (lo) atomicAdd((unsigned long long *)lo_p, (unsigned long long)lo);
     if (hi) atomicAdd((unsigned long long *)hi_p, (unsigned long long)hi);
 }
@@ -184,16 +215,19 @@ __device__ void tally_detection(const KArgs &A, const DevSet &S, uint32_t slot, 
     bool c1 = maxz > depth - S.half_lc;
     int cls = c1 ? 0 : 1;
     int B = S.num_bins;
-    uint64_t lo, hi, slo, shi;
+    uint32_t lo, slo2, slo;
+    uint64_t hi, shi;
     to_split(c, lo, hi);
-    to_split(c * c, slo, shi);
+    to_split3(c * c, slo2, slo, shi);
     uint64_t maxbits = (uint64_t)__double_as_longlong((double)c);
-    uint64_t *base = T + 4 + (uint64_t)cls * 6 * (uint64_t)B;
+    // Per class, B words each: Sum lo, Sum hi, SumSq lo, SumSq hi, count, Max, SumSq lo2.
+    uint64_t *base = T + 4 + (uint64_t)cls * 7 * (uint64_t)B;
     for (int ii = vp - 3; ii < vp + 3; ++ii) {
         if (ii >= 0 && ii < B) {
             if (ii == vp) atomicAdd((unsigned long long *)&T[1 + cls], 1ull);
             add_split(&base[ii], &base[B + ii], lo, hi);
             add_split(&base[2 * B + ii], &base[3 * B + ii], slo, shi);
+            if (slo2) atomicAdd((unsigned long long *)&base[6 * B + ii], (unsigned long long)slo2);
             atomicAdd((unsigned long long *)&base[4 * B + ii], 1ull);
             atomicMax((unsigned long long *)&base[5 * B + ii], (unsigned long long)maxbits);
         }
@@ -842,7 +876,8 @@ __global__ void fold_tallies_kernel(uint64_t *p, uint64_t n_words, uint64_t word
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t q = i % words;  // word within one A-scan's tally
-        const bool is_max = q >= 4 && (q - 4) % (6 * bins) >= 5 * bins;
+        const uint64_t r = q >= 4 ? (q - 4) % (7 * bins) : 0;
+        const bool is_max = q >= 4 && r >= 5 * bins && r < 6 * bins;
         uint64_t acc = p[i];
         for (uint32_t c = 1; c < copies; ++c) {
             const uint64_t v = p[i + c * stride];
@@ -855,7 +890,7 @@ __global__ void fold_tallies_kernel(uint64_t *p, uint64_t n_words, uint64_t word
 hipError_t launch_fold_tallies(uint64_t *p, uint64_t n_scans, int bins, uint32_t copies, uint64_t stride,
                                hipStream_t stream) {
     if (copies <= 1 || n_scans == 0) return hipSuccess;
-    const uint64_t words = 4 + 12 * (uint64_t)bins;
+    const uint64_t words = 4 + 14 * (uint64_t)bins;  // octmps_tally_words
     const uint64_t n = n_scans * words;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;

@@ -352,11 +352,12 @@ def decode_tally(t, bins):
     out = {"num_filtered": int(t[0]), "num_class1": int(t[1]), "num_class2": int(t[2])}
     s, sq, cnt, mx = [], [], [], []
     for c in range(2):
-        base = 4 + c * 6 * bins
-        blk = t[base:base + 6 * bins].reshape(6, bins)
-        # hi words in units of 2^-32, lo words in units of 2^-64 (include/octmps.h)
+        base = 4 + c * 7 * bins
+        blk = t[base:base + 7 * bins].reshape(7, bins)
+        # hi words in units of 2^-32, lo words in units of 2^-64, SumSq's lo2 in 2^-96 (include/octmps.h)
         s.append(blk[1].astype(np.float64) * 2.0 ** -32 + blk[0].astype(np.float64) * 2.0 ** -64)
-        sq.append(blk[3].astype(np.float64) * 2.0 ** -32 + blk[2].astype(np.float64) * 2.0 ** -64)
+        sq.append(blk[3].astype(np.float64) * 2.0 ** -32 + blk[2].astype(np.float64) * 2.0 ** -64
+                  + blk[6].astype(np.float64) * 2.0 ** -96)
         cnt.append(blk[4].astype(np.float64))
         mx.append(blk[5].view(np.float64).copy())
     out["sum"] = np.array(s)

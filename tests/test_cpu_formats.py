@@ -101,10 +101,10 @@ def _tally_from(sums, sqs, maxs, cnts, nf):
     """Integer tally whose sums are exactly the given doubles (multiples of 2^-30): the high
     word counts 2^-32 units, the low word 2^-64 units."""
     B = sums.shape[1]
-    t = np.zeros(4 + 12 * B, np.uint64)
+    t = np.zeros(4 + 14 * B, np.uint64)
     t[0:3] = nf
     for c in range(2):
-        base = 4 + c * 6 * B
+        base = 4 + c * 7 * B
         for arr, off in ((sums[c], 0), (sqs[c], 2)):
             k = np.rint(arr * 2.0 ** 30).astype(np.uint64)
             # value = k * 2^-30 = (4k) * 2^-32: all in the high word, low word 0

@@ -106,7 +106,7 @@ def test_tally_finalize_formula():
     n = 1000
     vals = {}
     for c in range(2):
-        base = 4 + c * 6 * B
+        base = 4 + c * 7 * B
         s = rng.integers(0, 2 ** 40, B).astype(np.uint64)
         q = (s.astype(np.float64) ** 2 / 2 ** 20 / n * rng.uniform(1, 3, B)).astype(np.uint64)
         # value s * 2^-20: high word (units 2^-32) = s * 2^12 - (low part), low word (2^-64 units)

@@ -342,9 +342,10 @@ def test_integration_python_example_runs(tmp_path, ctx, monkeypatch):
     out = ns["mean_se"]  # [filtered][class][mean, SE][bin]
     assert out.shape == (2, 2, 2, p.num_depth_bins) and np.all(np.isfinite(out[:, :, 0]))
     # SE is sqrt of (SumSq/N - mean^2)/(N-1) unclamped, as the reference prints it
-    # (src/octmps_io.cpp:385-388, 408-411). The fixed-point SumSq resolves 2^-64, so a
-    # contribution below 2.3e-10 adds nothing to it: a bin fed only by such contributions
-    # gets a negative variance, hence NaN. That happens only where the mean is below 1e-9.
+    # (src/octmps_io.cpp:385-388, 408-411). The fixed-point Sum resolves 2^-64 and SumSq
+    # 2^-96, so a contribution between 5.4e-20 and 3.6e-15 enters Sum but not SumSq: a bin
+    # fed only by such contributions gets a negative variance, hence NaN. That happens only
+    # where the mean is below 1e-9.
     se = out[:, :, 1]
     assert np.all(np.abs(out[:, :, 0][np.isnan(se)]) < 1e-9)
     assert np.isnan(se).mean() < 0.5
