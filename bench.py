@@ -4,11 +4,12 @@ bins to 1.5 mm), with the roofline of the transport kernel and the CPU oracle's 
 host cores beside it.
 
 A step is one full B-scan (all A-scans of the workload) through the batched path, launched
-as one hipGraph. The plain path (one A-scan per launch, synchronous) is timed after the steps
-on every 8th A-scan of the same B-scan. Then the CPU oracle runs a bounded sample of the
-same B-scan on the host cores, and a short served stream (Poisson arrivals over four
-resident meshes) gives p50/p99 request latency. Prints ONE JSON line on stdout; progress
-goes to stderr.
+as one hipGraph. The whole run keeps to a wall-clock budget (--budget-s): the fixed-cost
+legs run first (autotune on a 1/64 slice, warmup steps on a 1/64 slice, the plain path on
+every 8th A-scan, the CPU oracle on a bounded sample, the served stream and its load sweep,
+the cache and HBM probes), then as many of the requested timed steps as fit in what is left
+(at least one). `steps` reports the number timed; `steps_requested` the number asked for.
+Prints ONE JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
@@ -16,32 +17,45 @@ import os
 import sys
 import time
 
+T_START = time.time()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "oct-mps_amd"))
 
 ALG_BYTES_PER_STEP = 84  # one tet record per photon-step: 4 planes x 16 B + 4 links x 4 B + region 4 B
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E nominal
 LAYOUT_BYTES_PER_STEP = 80  # this build's packed tet record: 4 planes x 16 B + 4 packed neighbours x 4 B
-# Measured HBM bytes per photon-step of the batched slab4 launch: rocprofv3 --pmc FETCH_SIZE
-# (x2.0, the factor by which that counter under-reads a copy kernel of known size) plus
-# WRITE_SIZE (x1.0), divided by the launch's photon-steps. profiles/transport_slab4_pmc.md
-# holds the counter values and commands.
-TRAFFIC_B_PER_STEP = 26.09
-# Measured L1 lookups per photon-step of the batched slab4 launch (rocprofv3 --pmc
-# TCP_TOTAL_CACHE_ACCESSES_sum / photon-steps, profiles/transport_slab4_memory.md): the
-# 16-byte lane-loads of the tet record (5 per visit, 4 after entering through a face) plus
-# the other vector accesses. With octmps_probe_gather's ceiling for cached divergent record
-# gathers this gives the fraction of the gather path the kernel uses.
-LANE_LOADS_PER_STEP = 4.16
+
+# Counter-derived figures of the batched slab4 launch (128 A-scans x 1e6 photons), each from
+# the rocprofv3 summary named beside it; per photon-step = counter / the launch's photon-steps.
+PROFILE = {
+    # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors by which those counters read a copy
+    # kernel of known size), per photon-step.
+    "hbm_bytes_per_step": 26.09, "hbm_source": "profiles/transport_slab4_pmc.md",
+    # TD_TD_BUSY_sum / (CUs x launch cycles per CU); TA_BUSY_avr / launch cycles per CU.
+    "td_busy": 1.00, "ta_busy": 0.76, "mem_source": "profiles/transport_slab4_memory.md",
+    # SQ_INSTS_VALU x 4 cycles / (SIMDs x launch cycles): an upper bound on VALU issue.
+    "valu_issue_upper": 1.07, "valu_per_step": 5.58, "sq_source": "profiles/transport_slab4_sq.md",
+    # TCP_TOTAL_CACHE_ACCESSES_sum / photon-steps: 16-byte lane-loads of the tet record (5 per
+    # visit, 4 after entering through a face) plus the other vector accesses.
+    "lane_loads_per_step": 4.16,
+}
+
+
+def log(msg):
+    print(f"[bench {time.time() - T_START:6.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--budget-s", type=float, default=530.0,
+                    help="wall-clock budget of the whole run; timed steps stop when the next would pass it")
     ap.add_argument("--scans", type=int, default=128, help="A-scans in the B-scan (workload: 128)")
     ap.add_argument("--photons", type=int, default=1_000_000, help="photons per A-scan (workload: 1e6)")
+    ap.add_argument("--slice-div", type=int, default=64,
+                    help="warmup and autotune calibration use 1/k of every A-scan's photons")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-photons", type=int, default=3000, help="oracle photons per sampled A-scan")
@@ -51,8 +65,11 @@ def parse():
     ap.add_argument("--quanta", default="",
                     help="scatter,refill quanta to launch with instead of autotuning (e.g. 36,16: the same "
                          "launch a profiler run can then attribute to one kernel dispatch)")
-    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--serve-window", type=float, default=0.1, help="seconds of arrivals in the served stream")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate of the C3 stream, req/s")
+    ap.add_argument("--serve-window", type=float, default=0.1, help="seconds of arrivals in the C3 stream")
+    ap.add_argument("--sweep-rates", default="10,25,50,100,200",
+                    help="arrival rates (req/s) of the load sweep on the budget-scaled request mix")
+    ap.add_argument("--sweep-window", type=float, default=1.0, help="seconds of arrivals per sweep point")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-policy", type=int, default=0, choices=(0, 1),
                     help="request server hand-out order: 0 least remaining photons first, 1 equal shares")
@@ -68,18 +85,14 @@ def parse():
                          "in [1e5, 1e7] (seed 7); bump: 6.3M-tet bump mesh, 1024 A-scans x 1e6 photons")
     ap.add_argument("--stride", type=int, default=1,
                     help="keep every k-th A-scan of the workload (a bounded sample of a long B-scan)")
+    ap.add_argument("--no-probes", action="store_true", help="skip the gather and HBM copy probes")
     return ap.parse_args()
 
 
-def run_serving(O, configs, ctx, a, log):
-    """Served stream: Poisson arrivals (seed 11) over four resident meshes; each request is
-    1-256 A-scans with log-uniform budgets in [1e4, 1e6]. Latency = submit -> result."""
-    import threading
-
-    import numpy as np
-
+def resident_meshes(O, configs, ctx, a):
+    """The four meshes of the served stream (C3), uploaded once: sphere, slab4, ellipsoid and
+    the 128^3-cell layered slab."""
     meshes = []
-    t0 = time.time()
     m, p, s, _ = configs.sphere(photons=1)
     meshes.append((m, ctx.upload(m), p, s[0]))
     m, p, s, _ = configs.slab4(n_scans=1, photons=1)
@@ -90,47 +103,92 @@ def run_serving(O, configs, ctx, a, log):
     m, p = configs.slab128(cells=(c, c, c))
     root, z = m.find_root(0.0, 0.0, 0.0)
     meshes.append((m, ctx.upload(m), p, O.ascan(0.0, 0.0, z, root, 1)))
-    log(f"served meshes resident ({sum(x[0].num_tets for x in meshes)} tets) in {time.time() - t0:.1f}s")
-    trace = configs.served_stream(a.serve_rate, a.serve_window, seed=11, n_meshes=len(meshes))
-    srv = O.Server(ctx, policy=a.serve_policy)
-    ids = []
-    photons = 0
+    return meshes
+
+
+def serve_trace(O, srv, meshes, trace, salt0):
+    """Submits a Poisson trace at its arrival times and waits for every request. Returns the
+    per-request latencies (submit -> result, server clock, ms), the photons and the span."""
+    import numpy as np
+    ids, photons = [], 0
     start = time.perf_counter()
     for i, (t_arr, mi, n, budgets) in enumerate(trace):
         mesh, mid, p, probe = meshes[mi]
-        scans = [O.ascan(probe.x, probe.y, probe.z, probe.root_tet, int(b), salt=1000 * i + k)
+        scans = [O.ascan(probe.x, probe.y, probe.z, probe.root_tet, int(b), salt=salt0 + 1000 * i + k)
                  for k, b in enumerate(budgets)]
         photons += int(sum(budgets))
         delay = start + t_arr - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
         ids.append(srv.submit(mid, p, scans, 1))
-    lat = []
-    first = last = None
+    lat, first, last = [], None, None
     for rid in ids:
         info, _ = srv.wait(rid, -1.0)
-        assert info.status == 0
+        assert info.status == 0, info.status
+        srv.release(rid)
         lat.append(info.t_done_ms - info.t_submit_ms)
         first = info.t_submit_ms if first is None else min(first, info.t_submit_ms)
         last = info.t_done_ms if last is None else max(last, info.t_done_ms)
-    st = srv.stats()
-    # Latency floor: one small request (1 A-scan x 1e4 photons, sphere) on the idle server,
-    # five times.
-    mesh, mid, p, probe = meshes[0]
-    floor = []
-    for k in range(5):
-        small = [O.ascan(probe.x, probe.y, probe.z, probe.root_tet, 10000, salt=999000 + k)]
-        info, _ = srv.wait(srv.submit(mid, p, small, 1), -1.0)
-        floor.append(info.t_done_ms - info.t_submit_ms)
-    srv.close()
-    lat = np.array(lat)
-    span_s = (last - first) * 1e-3
-    return {"requests": len(ids), "rate_req_s": a.serve_rate, "arrival_window_s": a.serve_window,
-            "policy": ("least remaining photons first, aged", "equal shares")[a.serve_policy],
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "idle_small_request_ms": float(np.median(floor)),
-            "photons": photons, "photons_per_s": photons / span_s, "span_s": span_s,
-            "slices": st["slices"], "kernel_busy_frac": st["kernel_ms"] * 1e-3 / span_s,
+    return np.array(lat), photons, (last - first) * 1e-3
+
+
+def run_serving(O, configs, ctx, a):
+    """Served stream (C3): Poisson arrivals (seed 11) over four resident meshes; each request
+    is 1-256 A-scans with log-uniform budgets in [1e4, 1e6] (~2.8e7 photons per request, so
+    one GPU serves ~0.2 request/s of it and every C3 rate of 50-400 req/s is far past
+    saturation: latency there measures the queue). Beside it, a load sweep below saturation:
+    the same generator with budgets scaled by 1e-3 ([10, 1000] photons per A-scan)."""
+    import numpy as np
+    t0 = time.time()
+    meshes = resident_meshes(O, configs, ctx, a)
+    log(f"served meshes resident ({sum(x[0].num_tets for x in meshes)} tets) in {time.time() - t0:.1f}s")
+    srv = O.Server(ctx, policy=a.serve_policy)
+    try:
+        trace = configs.served_stream(a.serve_rate, a.serve_window, seed=11, n_meshes=len(meshes))
+        s0 = srv.stats()
+        lat, photons, span = serve_trace(O, srv, meshes, trace, 0)
+        s1 = srv.stats()
+        c3 = {"requests": len(lat), "rate_req_s": a.serve_rate, "arrival_window_s": a.serve_window,
+              "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+              "max_ms": float(lat.max()), "photons": photons, "photons_per_s": photons / span, "span_s": span,
+              "kernel_busy_frac": (s1["kernel_ms"] - s0["kernel_ms"]) * 1e-3 / span,
+              "note": f"p99 of {len(lat)} requests is their maximum; the mix averages ~2.8e7 photons per "
+                      "request, so every C3 rate (50-400 req/s) saturates one GPU"}
+        log(f"C3 stream: {len(lat)} requests, p50 {c3['p50_ms']:.0f} ms, p99 {c3['p99_ms']:.0f} ms, "
+            f"{c3['photons_per_s']:.3e} photons/s")
+        # Latency floor: one small request (1 A-scan x 1e4 photons, sphere) on the idle server.
+        mesh, mid, p, probe = meshes[0]
+        floor = []
+        for k in range(5):
+            small = [O.ascan(probe.x, probe.y, probe.z, probe.root_tet, 10000, salt=999000 + k)]
+            rid = srv.submit(mid, p, small, 1)
+            info, _ = srv.wait(rid, -1.0)
+            srv.release(rid)
+            floor.append(info.t_done_ms - info.t_submit_ms)
+        sweep = []
+        for k, rate in enumerate(float(r) for r in a.sweep_rates.split(",") if r):
+            tr = configs.served_stream(rate, a.sweep_window, seed=11, n_meshes=len(meshes), lo=10.0, hi=1000.0)
+            if not tr:
+                continue
+            s0 = srv.stats()
+            lat, photons, span = serve_trace(O, srv, meshes, tr, 10_000_000 * (k + 1))
+            s1 = srv.stats()
+            offered = photons / a.sweep_window
+            sweep.append({"rate_req_s": rate, "requests": len(lat), "p50_ms": float(np.percentile(lat, 50)),
+                          "p99_ms": float(np.percentile(lat, 99)), "photons_per_s": photons / span,
+                          "offered_photons_per_s": offered,
+                          "kernel_busy_frac": (s1["kernel_ms"] - s0["kernel_ms"]) * 1e-3 / span})
+            log(f"sweep {rate:g} req/s: {len(lat)} requests, p50 {sweep[-1]['p50_ms']:.0f} ms, "
+                f"p99 {sweep[-1]['p99_ms']:.0f} ms, busy {sweep[-1]['kernel_busy_frac']:.2f}")
+        st = srv.stats()
+    finally:
+        srv.close()
+    for m in meshes:
+        ctx.release(m[1])
+    return {"c3": c3, "policy": ("least remaining photons first, aged", "equal shares")[a.serve_policy],
+            "idle_small_request_ms": float(np.median(floor)), "slices": st["slices"],
+            "load_sweep": {"mix": "C3 generator (seed 11, 1-256 A-scans, 4 resident meshes) with per-A-scan "
+                                  "budgets log-uniform in [10, 1000] photons", "points": sweep},
             "meshes": "sphere 196,608 + slab4 786,432 + ellipsoid 786,432 + "
                       f"slab128 {meshes[3][0].num_tets} tets, all resident"}
 
@@ -146,6 +204,7 @@ def main():
     import octmps as O
     from octmps import configs
 
+    deadline = T_START + a.budget_s
     t0 = time.time()
     if a.workload == "slab4":
         mesh, params, scans, seed = configs.slab4(n_scans=a.scans, photons=a.photons)
@@ -160,7 +219,8 @@ def main():
         desc = (f"bump: 6x6x3 mm, 128x128x64 cells x 6 tets ({mesh.num_tets} tets), {{n}} of 1024 A-scans x "
                 f"1e6 photons, probes projected onto the bump, 1024 bins to 2 mm")
     scans = scans[::max(1, a.stride)]
-    print(f"[bench] mesh {mesh.num_tets} tets built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    total_photons = sum(s.n_photons for s in scans)
+    log(f"mesh {mesh.num_tets} tets built in {time.time() - t0:.1f}s")
     ctx = O.Context(0)
     mid = ctx.upload(mesh)
     copy = None
@@ -168,19 +228,19 @@ def main():
         nbytes = a.calibrate_copy_mb << 20
         ms = ctx.probe_copy(nbytes, 10)
         copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB_s": 2 * nbytes / (ms * 1e-3) / 1e9}
-        print(f"[bench] copy calibration: {nbytes} B in {ms:.3f} ms ({copy['GB_s']:.0f} GB/s r+w)",
-              file=sys.stderr, flush=True)
+        log(f"copy calibration: {nbytes} B in {ms:.3f} ms ({copy['GB_s']:.0f} GB/s r+w)")
     opts = O.BatchOpts()
     opts.use_graph = 1
     tuned = None
+    div = max(1, a.slice_div)
     if a.quanta:
         opts.scatter_quantum, opts.refill_quantum = (int(v) for v in a.quanta.split(","))
         tuned = {"scatter_quantum": opts.scatter_quantum, "refill_quantum": opts.refill_quantum, "source": "--quanta"}
     elif not a.no_autotune:
         # The autotuner (coordinate search over launch options on a calibration slice of the
-        # same B-scan: the first 1/16 of every A-scan's photons at the default size), before
-        # the timed region; results are the same bits under any options.
-        calib = max(1, sum(s.n_photons for s in scans) // 16)
+        # same B-scan: the first 1/div of every A-scan's photons), before the timed region;
+        # results are the same bits under any options.
+        calib = max(1, total_photons // div)
         ts = time.perf_counter()
         o, rate = ctx.autotune(mid, params, scans, seed, calib=calib)
         opts.block_size, opts.refill_quantum, opts.scatter_quantum = o.block_size, o.refill_quantum, o.scatter_quantum
@@ -188,35 +248,18 @@ def main():
         tuned = {"block_size": o.block_size, "refill_quantum": o.refill_quantum,
                  "scatter_quantum": o.scatter_quantum, "tally_copies": o.tally_copies, "entry_skip": o.entry_skip,
                  "calibration_photons": calib, "seconds": time.perf_counter() - ts}
-        print(f"[bench] autotune ({tuned['seconds']:.1f} s): block {o.block_size} scatter/refill quanta "
-              f"{o.scatter_quantum}/{o.refill_quantum} tally copies {o.tally_copies} entry skip {o.entry_skip} "
-              f"({rate:.3e} photons/s on the calibration slice)", file=sys.stderr, flush=True)
-    # Warmup on a small slice of the same B-scan (code object load, buffer allocation), then W
-    # full steps. The slice runs through the logged variant of the kernel, a different kernel
-    # symbol, so a profiler's per-kernel average of the unlogged single-set transport_kernel
-    # covers the timed launches only.
+        log(f"autotune ({tuned['seconds']:.1f} s): block {o.block_size} scatter/refill quanta "
+            f"{o.scatter_quantum}/{o.refill_quantum} tally copies {o.tally_copies} entry skip {o.entry_skip} "
+            f"({rate:.3e} photons/s on the calibration slice)")
+    # Warmup: the logged kernel variant once (a different kernel symbol, so a profiler's
+    # per-kernel average of the unlogged single-set transport_kernel covers the timed launches
+    # only), then W steps of the unlogged graph launch on a 1/div slice of the B-scan.
     small = [O.ascan(s.x, s.y, s.z, s.root_tet, min(s.n_photons, 500), s.stream_salt) for s in scans]
     ctx.run_bscan(mid, params, small, seed, opts=opts, log=True)
+    wslice = [O.ascan(s.x, s.y, s.z, s.root_tet, max(1, s.n_photons // div), s.stream_salt) for s in scans]
     for i in range(a.warmup):
-        ctx.run_bscan(mid, params, scans, seed, opts=opts)
-        print(f"[bench] warmup step {i} done", file=sys.stderr, flush=True)
-    total_photons = sum(s.n_photons for s in scans)
-    walls, kms, steps, iters = [], [], [], []
-    for i in range(a.steps):
-        ts = time.perf_counter()
-        t, st = ctx.run_bscan(mid, params, scans, seed, opts=opts)
-        walls.append(time.perf_counter() - ts)
-        kms.append(st.kernel_ms)
-        steps.append(st.photon_steps)
-        iters.append(st.wave_iterations)
-        print(f"[bench] step {i}: {walls[-1]:.3f}s kernel {st.kernel_ms:.1f} ms, {st.photon_steps} photon-steps, "
-              f"killed {st.killed}", file=sys.stderr, flush=True)
-    wall = sum(walls)
-    value = total_photons * a.steps / wall
-    kernel_s = sum(kms) * 1e-3
-    steps_per_launch = sum(steps) / len(steps)
-    achieved = steps_per_launch * ALG_BYTES_PER_STEP / (kernel_s / a.steps) / 1e9
-    batched_tally = t
+        _, st = ctx.run_bscan(mid, params, wslice, seed, opts=opts)
+        log(f"warmup step {i} (1/{div} slice): kernel {st.kernel_ms:.1f} ms")
 
     plain = None
     if not a.no_plain:
@@ -227,14 +270,14 @@ def main():
         ts = time.perf_counter()
         pk, pst, nph = 0.0, 0, 0
         for s in sub:
-            tp, stp = ctx.run_ascan(mid, params, s, seed)
+            tp, stp = ctx.run_ascan(mid, params, s, seed, opts=opts)
             pk += stp.kernel_ms
             pst += stp.photon_steps
             nph += stp.photons
         pw = time.perf_counter() - ts
         plain = {"photons_per_s": nph / pw, "wall_s": pw, "kernel_ms": pk, "photon_steps": pst,
                  "a_scans_timed": len(sub), "photons": nph}
-        print(f"[bench] plain: {len(sub)} A-scans in {pw:.3f}s", file=sys.stderr, flush=True)
+        log(f"plain: {len(sub)} A-scans in {pw:.3f}s ({nph / pw:.3e} photons/s)")
 
     cpu = None
     if not a.no_cpu:
@@ -277,62 +320,92 @@ def main():
                "single_thread_value": single, "cpu_model": model, "nproc": os.cpu_count(),
                "gpu_vs_oracle": {"photons": n, "reflectance_sum_gpu": gpu_sum, "reflectance_sum_oracle": oracle_sum,
                                  "rel_err": abs(gpu_sum - oracle_sum) / oracle_sum if oracle_sum else None}}
-        print(f"[bench] cpu oracle: {n} photons in {cw:.1f}s; same photons on the GPU: summed reflectance "
-              f"{gpu_sum:.6e} vs oracle {oracle_sum:.6e}", file=sys.stderr, flush=True)
+        log(f"cpu oracle: {n} photons in {cw:.1f}s; same photons on the GPU: summed reflectance "
+            f"{gpu_sum:.6e} vs oracle {oracle_sum:.6e}")
+
+    serving = None
+    if not a.no_serve:
+        serving = run_serving(O, configs, ctx, a)
 
     served_same = None
     if a.serve_same:
         # The same B-scan as one request to the request server (the multi-set kernel in
-        # wall-clock slices), after the timed region: its rate beside the batched launch and
-        # its tallies against the batched launch's, bit for bit.
+        # wall-clock slices): its rate beside the batched launch and its tallies against the
+        # batched launch's, bit for bit (the batched tallies come from the timed steps below).
         srv = O.Server(ctx)
         try:
             ts = time.perf_counter()
-            info, out = srv.wait(srv.submit(mid, params, scans, seed), -1.0)
+            rid = srv.submit(mid, params, scans, seed)
+            info, served_out = srv.wait(rid, -1.0)
             sw = time.perf_counter() - ts
             sst = srv.stats()
         finally:
             srv.close()
         served_same = {"photons_per_s": total_photons / sw, "wall_s": sw, "slices": sst["slices"],
-                       "status": info.status, "equal_to_batched": bool(np.array_equal(out, batched_tally))}
-        print(f"[bench] B-scan as one served request: {total_photons / sw:.3e} photons/s, "
-              f"equal to batched: {served_same['equal_to_batched']}", file=sys.stderr, flush=True)
+                       "status": info.status}
+        log(f"B-scan as one served request: {total_photons / sw:.3e} photons/s")
 
-    serving = None
-    if not a.no_serve:
-        serving = run_serving(O, configs, ctx, a, lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True))
-        print(f"[bench] served stream: {serving['requests']} requests, p50 {serving['p50_ms']:.1f} ms, "
-              f"p99 {serving['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
-
-    # Ceiling of the vector-memory path for divergent record gathers (measured after the
-    # timed region, on a cache-resident record set).
-    gather = None
-    if a.workload == "slab4":
-        peak = ctx.probe_gather()
-        got = sum(steps) / (sum(kms) * 1e-3) * LANE_LOADS_PER_STEP
-        gather = {"bound": "vector-memory gathers", "achieved": got, "peak": peak, "unit": "16-byte lane-loads/s",
-                  "frac": got / peak, "lane_loads_per_step": LANE_LOADS_PER_STEP,
-                  "peak_source": "octmps_probe_gather: 5 x 16 B from pseudo-random 128-byte records of a "
-                                 "2 MiB set, 6 waves per SIMD"}
-        print(f"[bench] gather ceiling {peak:.3e} lane-loads/s; kernel {got:.3e} ({got / peak:.2f})",
-              file=sys.stderr, flush=True)
-    # Measured HBM copy rate on this GPU, beside the nominal peak the roofline uses: 4 GiB
-    # read + 4 GiB written per copy (16x the Infinity Cache), after the timed region.
-    hbm_copy = None
-    if a.workload == "slab4":
+    # Ceiling of the vector-memory path for divergent record gathers (cache-resident record
+    # set), and the measured HBM copy rate beside the nominal peak the roofline uses: 4 GiB
+    # read + 4 GiB written per copy (16x the Infinity Cache).
+    gather_peak = hbm_copy = None
+    if not a.no_probes:
+        gather_peak = ctx.probe_gather()
         nb = 4 << 30
         ms = ctx.probe_copy(nb, 5)
         hbm_copy = 2 * nb / (ms * 1e-3) / 1e9
-        print(f"[bench] HBM copy: {hbm_copy:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
+        log(f"gather ceiling {gather_peak:.3e} lane-loads/s; HBM copy {hbm_copy:.0f} GB/s (read + write)")
+
+    # Timed steps: full B-scans, as many of the requested ones as fit in the budget.
+    walls, kms, steps, iters, bad = [], [], [], [], 0
+    t = None
+    for i in range(a.steps):
+        if walls:
+            est = max(walls) * 1.05 + 2.0
+            if time.time() + est > deadline:
+                log(f"budget: {i} of {a.steps} steps timed ({deadline - time.time():.0f} s left, a step takes "
+                    f"{max(walls):.1f} s)")
+                break
+        ts = time.perf_counter()
+        t, st = ctx.run_bscan(mid, params, scans, seed, opts=opts)
+        walls.append(time.perf_counter() - ts)
+        kms.append(st.kernel_ms)
+        steps.append(st.photon_steps)
+        iters.append(st.wave_iterations)
+        bad += st.bad_contributions
+        log(f"step {i}: {walls[-1]:.3f}s kernel {st.kernel_ms:.1f} ms, {st.photon_steps} photon-steps, "
+            f"killed {st.killed}")
+    n_steps = len(walls)
+    wall = sum(walls)
+    value = total_photons * n_steps / wall
+    kernel_s = sum(kms) * 1e-3
+    steps_per_launch = sum(steps) / n_steps
+    launch_s = kernel_s / n_steps
+    achieved = steps_per_launch * ALG_BYTES_PER_STEP / launch_s / 1e9
+    hbm_measured = steps_per_launch * PROFILE["hbm_bytes_per_step"] / launch_s / 1e9
+    lane_util = sum(steps) / (64.0 * max(1, sum(iters)))
+    if served_same is not None:
+        served_same["equal_to_batched"] = bool(np.array_equal(served_out, t))
+
+    gather = None
+    if gather_peak:
+        got = sum(steps) / kernel_s * PROFILE["lane_loads_per_step"]
+        gather = {"bound": "vector-memory gathers", "achieved": got, "peak": gather_peak,
+                  "unit": "16-byte lane-loads/s", "frac": got / gather_peak,
+                  "lane_loads_per_step": PROFILE["lane_loads_per_step"],
+                  "peak_source": "octmps_probe_gather: 5 x 16 B from pseudo-random 128-byte records of a "
+                                 "2 MiB set, 6 waves per SIMD"}
     ctx.close()
+    frac = achieved / HBM_PEAK_GBS
     line = {
         "metric": "photons/s per B-scan, batched vs plain kernel; p50/p99 A-scan request latency",
         "value": value,
         "unit": "photons/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": n_steps,
+        "steps_requested": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
+        "ms_per_step": wall / n_steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -340,23 +413,36 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc.format(n=len(scans)),
                    "photons": total_photons, "path": "batched persistent kernel, hipGraph launch",
-                   "launch_options": tuned or "context defaults"},
+                   "launch_options": tuned or "context defaults",
+                   "warmup_slice": f"1/{div} of every A-scan's photons"},
         "batched": {"photons_per_s": value, "kernel_ms": kms, "photon_steps": steps,
-                    "steps_per_photon": steps_per_launch / total_photons,
-                    "lane_utilisation": sum(steps) / (64.0 * max(1, sum(iters)))},
+                    "steps_per_photon": steps_per_launch / total_photons, "lane_utilisation": lane_util,
+                    "bad_contributions": bad},
         "plain": plain,
         "copy_calibration": copy,
         "batched_over_plain": (value / plain["photons_per_s"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (steps_per_launch * TRAFFIC_B_PER_STEP) if TRAFFIC_B_PER_STEP else None,
+                     "frac": frac,
+                     "traffic": steps_per_launch * PROFILE["hbm_bytes_per_step"],
                      "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step",
+                     "binding": frac <= 1.0,
+                     "note": ("algorithmic bytes (one tet record per photon-step) over the nominal HBM peak; "
+                              "above 1 because the mesh is served mostly from L2 and the Infinity Cache, so HBM "
+                              "does not bind" if frac > 1.0 else
+                              "algorithmic bytes (one tet record per photon-step) over the nominal HBM peak"),
                      "peak_kind": "nominal MI355X HBM3E", "measured_copy_GB_s": hbm_copy,
-                     "layout_bytes_per_step": LAYOUT_BYTES_PER_STEP},
+                     "layout_bytes_per_step": LAYOUT_BYTES_PER_STEP,
+                     "measured_hbm": {"bytes_per_step": PROFILE["hbm_bytes_per_step"], "GB_s": hbm_measured,
+                                      "frac": hbm_measured / HBM_PEAK_GBS, "source": PROFILE["hbm_source"]}},
+        "bound_by": {"resource": "TD (vector-memory data return of the divergent tet-record gathers)",
+                     "frac": PROFILE["td_busy"], "source": PROFILE["mem_source"],
+                     "ta_busy": PROFILE["ta_busy"], "valu_issue_upper_bound": PROFILE["valu_issue_upper"],
+                     "valu_source": PROFILE["sq_source"], "lane_utilisation": lane_util},
         "gather_roofline": gather,
         "cpu_baseline": cpu,
         "serving": serving,
         "served_same_bscan": served_same,
+        "wall_s": time.time() - T_START,
     }
     print(json.dumps(line), flush=True)
 

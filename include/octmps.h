@@ -172,18 +172,24 @@ octmps_status octmps_photon_streams(uint64_t seed, uint64_t salt, uint64_t photo
 
 /* ------------------------------------------------------------------------- */
 /* Tallies: SimState (src/octmps.h:133-176) as exact integers.                */
-/* One A-scan tally is octmps_tally_words(B) uint64 words:                    */
-/*   [0] NumFilteredPhotons [1] ...ClassI [2] ...ClassII [3] reserved          */
-/*   then for class c in {0: Class I, 1: Class II}, base = 4 + c*7*B:          */
-/*   base+0*B sum_lo, +1*B sum_hi   Sum = sum_hi * 2^-32 + sum_lo * 2^-64      */
-/*   base+2*B sq_lo,  +3*B sq_hi,  +6*B sq_lo2                                 */
-/*                 SumSq = sq_hi * 2^-32 + sq_lo * 2^-64 + sq_lo2 * 2^-96     */
-/*   (every contribution c adds floor(c*2^64) split as hi*2^32 + lo, lo < 2^32, */
-/*    and c^2 adds floor(c^2*2^96) split the same way over three words: exact  */
-/*    and carry-free for up to 2^32 contributions per bin)                     */
-/*   base+4*B count (NumClass*_PhotonsFilteredInRange)                         */
-/*   base+5*B max   (IEEE-754 binary64 bits of Max)                            */
+/* One A-scan tally is octmps_tally_words(B) = 4 + 2*29*B uint64 words:       */
+/*   [0] NumFilteredPhotons [1] ...ClassI [2] ...ClassII                      */
+/*   [3] contributions this tally could not hold (not finite or negative; 0)  */
+/*   then for class c in {0: Class I, 1: Class II}, base = 4 + c*29*B:         */
+/*   base + k*B,      k < 9:  Sum digit k   Sum   = sum_k S_k * 2^(32k - 160)  */
+/*   base + (9+k)*B,  k < 18: SumSq digit k SumSq = sum_k Q_k * 2^(32k - 320)  */
+/*   base + 27*B count (NumClass*_PhotonsFilteredInRange)                      */
+/*   base + 28*B max   (IEEE-754 binary64 bits of Max)                          */
+/* A contribution c = m * 2^(e-150) (FP32 fields) adds m << (e+10) to Sum and   */
+/* m*m << (2e+20) to SumSq, cut into 32-bit pieces, each added to its digit     */
+/* word: every positive finite FP32 c and its square are held exactly, with no  */
+/* carry between words, for up to 2^32 contributions per bin.                   */
 /* ------------------------------------------------------------------------- */
+#define OCTMPS_TALLY_SUM_DIGITS 9
+#define OCTMPS_TALLY_SQ_DIGITS 18
+#define OCTMPS_TALLY_CLASS_WORDS 29   /* per bin and class: 9 + 18 digits, count, max */
+#define OCTMPS_TALLY_SUM_EXP (-160)   /* Sum digit 0 counts units of 2^-160 */
+#define OCTMPS_TALLY_SQ_EXP (-320)    /* SumSq digit 0 counts units of 2^-320 */
 uint64_t octmps_tally_words(int32_t num_bins);
 
 /* Mean and standard error per bin, raw and filtered
@@ -235,6 +241,8 @@ typedef struct {
     uint32_t refill_quantum;
     uint64_t wave_iterations; /* loop iterations summed over waves; lane utilisation =
                                  photon_steps / (64 * wave_iterations) */
+    uint64_t bad_contributions; /* detector contributions the tally could not hold exactly
+                                   (expected 0; see the tally layout above) */
 } octmps_run_stats;
 
 typedef struct {
@@ -329,6 +337,10 @@ octmps_status octmps_serve_poll(octmps_server *s, uint64_t req_id, octmps_reques
 octmps_status octmps_serve_wait(octmps_server *s, uint64_t req_id, double timeout_s,
                                 octmps_request_info *info);
 octmps_status octmps_serve_cancel(octmps_server *s, uint64_t req_id);
+/* Forget a finished request (its id becomes unknown and its record is freed), so a
+ * long-running server does not keep every request it has served. OCTMPS_E_BUSY while the
+ * request is still running. No counterpart in the reference. */
+octmps_status octmps_serve_release(octmps_server *s, uint64_t req_id);
 /* Aggregate counters since open: photons, photon-steps, busy kernel ms, slices. */
 octmps_status octmps_serve_stats(octmps_server *s, uint64_t *photons, uint64_t *steps,
                                  double *kernel_ms, uint64_t *slices);
@@ -351,6 +363,15 @@ octmps_status octmps_probe_fresnel(octmps_ctx *ctx, int32_t n, const float *ni, 
  * for uniform draws u in (0,1]. */
 octmps_status octmps_probe_sampler(octmps_ctx *ctx, int32_t kind, float param, int32_t n,
                                    const float *u, float *cost_out);
+
+/* The device's detector tally (FastReflectTransmit's tally, src/octmps_kernel.cuh:684-741)
+ * run on n chosen detections (optical path op[i], max depth maxz[i], contribution
+ * contrib[i]) into one zeroed A-scan tally of p's bins: tally_out has
+ * octmps_tally_words(p->num_depth_bins) words; *bad_out (may be NULL) gets the number of
+ * contributions the tally could not hold. */
+octmps_status octmps_probe_tally(octmps_ctx *ctx, const octmps_params *p, int32_t n, const float *op,
+                                 const float *maxz, const float *contrib, uint64_t *tally_out,
+                                 uint64_t *bad_out);
 
 /* Device-to-device copy of `bytes` through a plain streaming kernel (no counterpart in the
  * reference): the calibration load for hardware byte counters. Returns the kernel time. */

@@ -133,7 +133,8 @@ struct PhotonLogRec {
 };
 
 // Control words.
-enum { CTRL_NEXT_ID = 0, CTRL_STEPS = 1, CTRL_KILLED = 2, CTRL_DONE = 3, CTRL_ITERS = 4, CTRL_WORDS = 8 };
+// CTRL_BAD counts detector contributions the tally could not hold (see tally_detection).
+enum { CTRL_NEXT_ID = 0, CTRL_STEPS = 1, CTRL_KILLED = 2, CTRL_BAD = 3, CTRL_ITERS = 4, CTRL_WORDS = 8 };
 
 struct KArgs {
     const TetRec *tets;
@@ -175,7 +176,7 @@ struct KArgs {
 hipError_t launch_transport(const KArgs &a, int grid, int block, bool multi, bool log,
                             hipStream_t stream);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t stream);
-// Folds copies 1..copies-1 of n_scans tallies (each 4 + 12*bins words, copy c at
+// Folds copies 1..copies-1 of n_scans tallies (each octmps_tally_words(bins) words, copy c at
 // p + c*stride) into copy 0: sums, except the Max words, which take the maximum.
 hipError_t launch_fold_tallies(uint64_t *p, uint64_t n_scans, int bins, uint32_t copies, uint64_t stride,
                                hipStream_t stream);
@@ -189,6 +190,9 @@ hipError_t launch_probe_fresnel(int n, const float *ni, const float *nt, const f
 hipError_t launch_probe_sampler(int kind, float param, int n, const float *u, float *out,
                                 hipStream_t stream);
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
+// n detections (op[i], maxz[i], c[i]) into the tally of a.slots[0] with parameter set S.
+hipError_t launch_probe_tally(const KArgs &a, const DevSet &S, int n, const float *op, const float *maxz,
+                              const float *c, hipStream_t stream);
 // blocks x 256 lanes each read iters records (5 x 16 B) of recs[0..n).
 hipError_t launch_gather_probe(const TetRec *recs, uint32_t n, int iters, uint32_t *out, int blocks,
                                hipStream_t stream);

@@ -57,6 +57,6 @@ void photon_streams(const StreamTable &t, uint64_t seed, uint64_t salt, uint64_t
                     uint64_t x[3], uint32_t a[3]);
 
 // Tally layout helpers (see octmps.h).
-inline uint64_t tally_words(int32_t b) { return 4ull + 14ull * (uint64_t)b; }
+inline uint64_t tally_words(int32_t b) { return 4ull + 2ull * OCTMPS_TALLY_CLASS_WORDS * (uint64_t)b; }
 
 }  // namespace octmps

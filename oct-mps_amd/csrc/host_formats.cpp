@@ -233,23 +233,86 @@ extern "C" uint64_t octmps_tally_words(int32_t b) { return b > 0 ? tally_words(b
 
 namespace {
 
-// Tally word pair: hi in units of 2^-32, lo in units of 2^-64 (see octmps.h).
-long double split_to_ld(uint64_t lo, uint64_t hi) {
-    return (long double)hi * 0x1p-32L + (long double)lo * 0x1p-64L;
+// Exact value of one digit accumulator (include/octmps.h): value = sum_k w[k*B] * 2^(32k + exp0).
+// Carries are propagated into 32-bit limbs, so the integer is held exactly; it is turned into
+// a double only at the end (FP64 final normalisation).
+struct BigNum {
+    static constexpr int kLimbs = 24;  // 18 digits of up to 64 bits need at most 20 limbs
+    uint32_t limb[kLimbs];
+    int exp0;  // binary exponent of limb 0's unit
+
+    BigNum(const uint64_t *w, int digits, uint64_t stride, int e0) : exp0(e0) {
+        std::memset(limb, 0, sizeof limb);
+        unsigned __int128 carry = 0;
+        for (int k = 0; k < kLimbs; ++k) {
+            unsigned __int128 t = carry + (k < digits ? (unsigned __int128)w[(uint64_t)k * stride] : 0);
+            limb[k] = (uint32_t)t;
+            carry = t >> 32;
+        }
+    }
+    // Subtracts m << shift (m < 2^64) from the integer; returns false (and leaves 0) if the
+    // integer is smaller.
+    bool sub(uint64_t m, int shift) {
+        uint32_t d[kLimbs] = {};
+        const int k = shift >> 5, r = shift & 31;
+        const unsigned __int128 v = (unsigned __int128)m << r;
+        for (int j = 0; j < 4 && k + j < kLimbs; ++j) d[k + j] = (uint32_t)(v >> (32 * j));
+        int64_t borrow = 0;
+        for (int j = 0; j < kLimbs; ++j) {
+            int64_t t = (int64_t)limb[j] - (int64_t)d[j] - borrow;
+            borrow = t < 0;
+            limb[j] = (uint32_t)(t + (borrow << 32));
+        }
+        if (borrow) std::memset(limb, 0, sizeof limb);
+        return !borrow;
+    }
+    double value() const {
+        int h = kLimbs - 1;
+        while (h >= 0 && limb[h] == 0) --h;
+        if (h < 0) return 0.0;
+        long double v = 0;
+        const int lo = h >= 3 ? h - 3 : 0;
+        for (int k = h; k >= lo; --k) v = v * 4294967296.0L + (long double)limb[k];
+        return (double)ldexpl(v, 32 * lo + exp0);
+    }
+};
+
+// FP32 fields of a contribution stored as binary64 (Max): c = m * 2^(e-150).
+void fp32_fields(double c, uint64_t &m, int &e) {
+    uint32_t b;
+    float f = (float)c;
+    std::memcpy(&b, &f, 4);
+    e = (int)((b >> 23) & 0xffu);
+    m = b & 0x7fffffu;
+    if (e == 0) e = 1;
+    else m |= 0x800000u;
 }
 
 struct Bin {
     double sum, sumsq, max;
+    double fsum, fsumsq;  // Sum - Max and SumSq - Max^2, exact before rounding
     uint64_t count;
 };
 
 Bin get_bin(const uint64_t *t, int32_t B, int c, int b) {
-    const uint64_t *base = t + 4 + (uint64_t)c * 7 * B;
+    const uint64_t *base = t + 4 + (uint64_t)c * OCTMPS_TALLY_CLASS_WORDS * B;
+    BigNum s(base + b, OCTMPS_TALLY_SUM_DIGITS, (uint64_t)B, OCTMPS_TALLY_SUM_EXP);
+    BigNum q(base + (uint64_t)OCTMPS_TALLY_SUM_DIGITS * B + b, OCTMPS_TALLY_SQ_DIGITS, (uint64_t)B,
+             OCTMPS_TALLY_SQ_EXP);
     Bin r;
-    r.sum = (double)split_to_ld(base[b], base[B + b]);
-    r.sumsq = (double)(split_to_ld(base[2 * B + b], base[3 * B + b]) + (long double)base[6 * B + b] * 0x1p-96L);
-    r.count = base[4 * B + b];
-    std::memcpy(&r.max, &base[5 * B + b], sizeof(double));
+    r.sum = s.value();
+    r.sumsq = q.value();
+    r.count = base[(uint64_t)(OCTMPS_TALLY_CLASS_WORDS - 2) * B + b];
+    std::memcpy(&r.max, &base[(uint64_t)(OCTMPS_TALLY_CLASS_WORDS - 1) * B + b], sizeof(double));
+    uint64_t m;
+    int e;
+    fp32_fields(r.max, m, e);
+    if (r.max > 0) {
+        s.sub(m, e + 10);
+        q.sub(m * m, 2 * e + 20);
+    }
+    r.fsum = r.max > 0 ? s.value() : r.sum;
+    r.fsumsq = r.max > 0 ? q.value() : r.sumsq;
     return r;
 }
 
@@ -266,8 +329,8 @@ extern "C" octmps_status octmps_tally_finalize(const uint64_t *tally, int32_t B,
             for (int f = 0; f < 2; ++f) {
                 double sum = x.sum, sq = x.sumsq;
                 if (f == 1 && x.max > mrc * sum) {
-                    sum -= x.max;
-                    sq -= x.max * x.max;
+                    sum = x.fsum;
+                    sq = x.fsumsq;
                 }
                 double mean = sum / N;
                 double var = (sq / N - mean * mean) / (double)(n - 1);
@@ -327,8 +390,8 @@ extern "C" octmps_status octmps_write_ascan_outputs(const char *dir, const octmp
                 Bin x = get_bin(tally, B, c, (int)b);
                 double sum = x.sum, sq = x.sumsq;
                 if (f == 1 && x.max > p->max_relative_contribution * sum) {
-                    sum -= x.max;
-                    sq -= x.max * x.max;
+                    sum = x.fsum;
+                    sq = x.fsumsq;
                 }
                 double mean = sum / nph;
                 double var = (sq / nph - mean * mean) / (nph - 1);

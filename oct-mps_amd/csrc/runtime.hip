@@ -314,6 +314,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
         stats->grid_size = (uint32_t)grid;
         stats->refill_quantum = (uint32_t)q;
         stats->wave_iterations = ctrl[CTRL_ITERS];
+        stats->bad_contributions = ctrl[CTRL_BAD];
     }
     return OCTMPS_OK;
 }
@@ -620,6 +621,49 @@ extern "C" octmps_status octmps_probe_sampler(octmps_ctx *c, int32_t kind, float
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d);
     return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_sampler");
+}
+
+extern "C" octmps_status octmps_probe_tally(octmps_ctx *c, const octmps_params *p, int32_t n, const float *op,
+                                            const float *maxz, const float *contrib, uint64_t *tally_out,
+                                            uint64_t *bad_out) {
+    if (!c || !p || n <= 0 || !op || !maxz || !contrib || !tally_out) return fail(OCTMPS_E_INVALID, "bad probe arguments");
+    DevSet S;
+    std::vector<DevRegion> R(kMaxRegions);
+    octmps_status st = make_set(p, 1, 0, S, R.data());
+    if (st != OCTMPS_OK) return st;
+    std::lock_guard<std::mutex> lk(c->run_mu);
+    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
+    const uint64_t words = tally_words(p->num_depth_bins);
+    DevScan scan;
+    std::memset(&scan, 0, sizeof scan);
+    // Layout of one device block: inputs (3 n floats), the scan slot, ctrl, the tally.
+    const size_t in_bytes = ((size_t)3 * n * sizeof(float) + 255) & ~(size_t)255;
+    const size_t bytes = in_bytes + sizeof(DevScan) + CTRL_WORDS * 8 + words * 8;
+    char *d = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&d, bytes)) != hipSuccess) return hip_fail(e, "probe allocation");
+    float *dop = (float *)d, *dmz = dop + n, *dc = dmz + n;
+    DevScan *dscan = (DevScan *)(d + in_bytes);
+    uint64_t *dctrl = (uint64_t *)(dscan + 1), *dt = dctrl + CTRL_WORDS;
+    KArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.slots = dscan;
+    A.ctrl = dctrl;
+    A.tallies = dt;
+    A.tally_copies = 1;
+    uint64_t ctrl[CTRL_WORDS];
+    e = hipMemcpyAsync(dop, op, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dmz, maxz, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dc, contrib, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dscan, &scan, sizeof scan, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_fill_u64(dctrl, CTRL_WORDS + words, 0, c->stream);
+    if (e == hipSuccess) e = launch_probe_tally(A, S, n, dop, dmz, dc, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tally_out, dt, words * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctrl, dctrl, sizeof ctrl, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e == hipSuccess && bad_out) *bad_out = ctrl[CTRL_BAD];
+    return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_tally");
 }
 
 extern "C" octmps_status octmps_probe_gather(octmps_ctx *c, int32_t n_records, int32_t iters,

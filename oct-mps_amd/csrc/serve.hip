@@ -567,6 +567,17 @@ extern "C" octmps_status octmps_serve_cancel(octmps_server *s, uint64_t id) {
     return OCTMPS_OK;
 }
 
+extern "C" octmps_status octmps_serve_release(octmps_server *s, uint64_t id) {
+    if (!s) return fail(OCTMPS_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(OCTMPS_E_NOT_FOUND, "unknown request");
+    if (!it->second->done) return fail(OCTMPS_E_BUSY, "request still running");
+    delete it->second;
+    s->reqs.erase(it);
+    return OCTMPS_OK;
+}
+
 extern "C" octmps_status octmps_serve_stats(octmps_server *s, uint64_t *photons, uint64_t *steps, double *kernel_ms,
                                             uint64_t *slices) {
     if (!s) return fail(OCTMPS_E_INVALID, "null argument");

@@ -8,7 +8,7 @@
 // three MWC streams from (seed, salt, photon index), so results do not depend on lane,
 // launch or slice. Transport is FP32, compiled without FP contraction: every fused
 // multiply-add is written as fmaf, so each kernel variant rounds identically. Tallies are
-// exact fixed-point integers (two 64-bit words per value with a carry-free split,
+// exact fixed-point integers (32-bit digit pieces in 64-bit words, carry-free,
 // non-returning atomics; the reference's EMULATED_ATOMIC, src/octmps_kernel.cuh:42-46,
 // needed a returning add to propagate its carry).
 #include <hip/hip_runtime.h>
@@ -17,6 +17,7 @@
 #include <cmath>
 
 #include "device_types.hpp"
+#include "octmps.h"
 #include "stream_init.hpp"
 
 namespace octmps {
@@ -139,70 +140,50 @@ __device__ __forceinline__ void rotate(float ax, float ay, float az, float cost,
     uz = nz * rn;
 }
 
-// Exact split fixed point of a non-negative float: v = floor(c * 2^64) = hi * 2^32 + lo with
-// lo < 2^32. The tally keeps two u64 accumulators, sum(hi) in units of 2^-32 and sum(lo) in
-// units of 2^-64; since each lo is < 2^32 neither word needs a carry, so both adds are
-// non-returning atomics and the total is exact for up to 2^32 adds per bin.
-__device__ __forceinline__ void to_split(float c, uint32_t &lo, uint64_t &hi) {
-    lo = 0;
-    hi = 0;
-    if (!(c > 0.0f) || !(c < 4.0e9f)) return;
-    uint32_t b = __float_as_uint(c);
+// Exact fixed-point pieces of a detector contribution c (FP32, c = m * 2^(e-150) with the
+// 24-bit significand m): Sum gets m << (e + 10) in units of 2^-160 and SumSq gets
+// m*m << (2e + 20) in units of 2^-320, cut at 32-bit boundaries into pieces that go to
+// consecutive digit words (include/octmps.h). Every positive finite FP32 value and its
+// square are then held exactly: Sum spans 2^-149 .. 2^128 in 9 digits and SumSq 2^-298 ..
+// 2^256 in 18. A piece is < 2^32 and no carry crosses words, so each add is a
+// non-returning atomic and a word cannot wrap before 2^32 adds.
+struct ExactPieces {
+    int ks, kq;            // first Sum digit, first SumSq digit
+    uint32_t s0, s1;       // Sum pieces for digits ks, ks + 1
+    uint32_t q0, q1, q2;   // SumSq pieces for digits kq, kq + 1, kq + 2
+};
+
+__device__ __forceinline__ ExactPieces exact_pieces(float c) {
+    ExactPieces P;
+    const uint32_t b = __float_as_uint(c);
     int e = (int)((b >> 23) & 0xffu);
-    uint64_t m = b & 0x7fffffu;
-    if (e == 0) e = 1;
+    uint32_t m = b & 0x7fffffu;
+    if (e == 0) e = 1;  // subnormal: no implicit bit, exponent of the smallest normal
     else m |= 0x800000u;
-    int sh = e - 86;  // c * 2^64 = m * 2^(e - 150 + 64), m < 2^24
-    if (sh <= -24) return;
-    if (sh < 0) {
-        lo = (uint32_t)(m >> (-sh));
-    } else if (sh < 32) {
-        uint64_t v = m << sh;  // < 2^56
-        lo = (uint32_t)v;
-        hi = v >> 32;
-    } else {
-        hi = m << (sh - 32);
-    }
+    const int sh = e + 10;        // 11 .. 264
+    P.ks = sh >> 5;               // 0 .. 8; s1 != 0 needs (sh & 31) >= 9, so ks + 1 <= 8
+    const uint64_t v = (uint64_t)m << (sh & 31);
+    P.s0 = (uint32_t)v;
+    P.s1 = (uint32_t)(v >> 32);
+    const uint64_t M = (uint64_t)m * (uint64_t)m;  // < 2^48
+    const int sh2 = 2 * e + 20;   // 22 .. 528
+    const int r2 = sh2 & 31;
+    P.kq = sh2 >> 5;              // 0 .. 16; q2 != 0 needs r2 >= 17, so kq + 2 <= 17
+    const uint64_t lo = M << r2;
+    P.q0 = (uint32_t)lo;
+    P.q1 = (uint32_t)(lo >> 32);
+    P.q2 = r2 ? (uint32_t)(M >> (64 - r2)) : 0u;
+    return P;
 }
 
-// The same fixed point 32 bits finer, for SumSq: v = floor(x * 2^96) = w2 * 2^64 + w1 * 2^32 + w0
-// with w0, w1 < 2^32; w2 counts 2^-32 units, w1 2^-64 units (the two words to_split gives) and
-// w0 2^-96 units, so a contribution c down to 3.6e-15 still adds its c^2.
-__device__ __forceinline__ void to_split3(float x, uint32_t &w0, uint32_t &w1, uint64_t &w2) {
-    w0 = 0;
-    w1 = 0;
-    w2 = 0;
-    if (!(x > 0.0f) || !(x < 4.0e9f)) return;
-    uint32_t b = __float_as_uint(x);
-    int e = (int)((b >> 23) & 0xffu);
-    uint64_t m = b & 0x7fffffu;
-    if (e == 0) e = 1;
-    else m |= 0x800000u;
-    int sh = e - 54;  // x * 2^96 = m * 2^(e - 150 + 96), m < 2^24, sh <= 104
-    if (sh <= -24) return;
-    uint64_t lo64, hi64;  // v = hi64 * 2^64 + lo64
-    if (sh < 0) {
-        lo64 = m >> (-sh);
-        hi64 = 0;
-    } else if (sh < 64) {
-        lo64 = m << sh;
-        hi64 = sh > 40 ? m >> (64 - sh) : 0;
-    } else {
-        lo64 = 0;
-        hi64 = m << (sh - 64);
-    }
-    w0 = (uint32_t)lo64;
-    w1 = (uint32_t)(lo64 >> 32);
-    w2 = hi64;
-}
-
-__device__ __forceinline__ void add_split(uint64_t *lo_p, uint64_t *hi_p, uint32_t lo, uint64_t hi) {
-    if (lo) atomicAdd((unsigned long long *)lo_p, (unsigned long long)lo);
-    if (hi) atomicAdd((unsigned long long *)hi_p, (unsigned long long)hi);
+__device__ __forceinline__ void add_piece(uint64_t *p, uint32_t v) {
+    if (v) atomicAdd((unsigned long long *)p, (unsigned long long)v);
 }
 
 // Detector tally, src/octmps_kernel.cuh:684-741: six bins around the optical-depth bin,
-// Class I when MaxDepth > OP/2 - Lc/2.
+// Class I when MaxDepth > OP/2 - Lc/2. A contribution that is not finite (an overflowed
+// likelihood ratio) cannot be held: it is counted in word 3 of the tally and in
+// ctrl[CTRL_BAD], and the bins still count the detection.
 __device__ void tally_detection(const KArgs &A, const DevSet &S, uint32_t slot, float op, float maxz,
                                 float c) {
     // Tally privatisation: workgroup w adds into copy w % copies (folded after the launch).
@@ -214,22 +195,30 @@ __device__ void tally_detection(const KArgs &A, const DevSet &S, uint32_t slot, 
     int vp = q <= 0.0f ? 0 : (q >= 1.0e9f ? 1000000000 : (int)q);
     bool c1 = maxz > depth - S.half_lc;
     int cls = c1 ? 0 : 1;
-    int B = S.num_bins;
-    uint32_t lo, slo2, slo;
-    uint64_t hi, shi;
-    to_split(c, lo, hi);
-    to_split3(c * c, slo2, slo, shi);
-    uint64_t maxbits = (uint64_t)__double_as_longlong((double)c);
-    // Per class, B words each: Sum lo, Sum hi, SumSq lo, SumSq hi, count, Max, SumSq lo2.
-    uint64_t *base = T + 4 + (uint64_t)cls * 7 * (uint64_t)B;
+    const uint64_t B = (uint64_t)S.num_bins;
+    const bool held = c >= 0.0f && c <= FLT_MAX;  // false for NaN and infinity
+    if (!held) {
+        atomicAdd((unsigned long long *)&T[3], 1ull);
+        atomicAdd((unsigned long long *)&A.ctrl[CTRL_BAD], 1ull);
+    }
+    ExactPieces P = exact_pieces(held ? c : 0.0f);
+    if (!(c > 0.0f) || !held) P.s0 = P.s1 = P.q0 = P.q1 = P.q2 = 0u;
+    const uint64_t maxbits = (uint64_t)__double_as_longlong((double)(held ? c : 0.0f));
+    // Per class, B words each: 9 Sum digits, 18 SumSq digits, count, Max.
+    uint64_t *base = T + 4 + (uint64_t)cls * OCTMPS_TALLY_CLASS_WORDS * B;
+    uint64_t *sum = base + (uint64_t)P.ks * B;
+    uint64_t *sq = base + (uint64_t)(OCTMPS_TALLY_SUM_DIGITS + P.kq) * B;
     for (int ii = vp - 3; ii < vp + 3; ++ii) {
-        if (ii >= 0 && ii < B) {
+        if (ii >= 0 && ii < (int)B) {
             if (ii == vp) atomicAdd((unsigned long long *)&T[1 + cls], 1ull);
-            add_split(&base[ii], &base[B + ii], lo, hi);
-            add_split(&base[2 * B + ii], &base[3 * B + ii], slo, shi);
-            if (slo2) atomicAdd((unsigned long long *)&base[6 * B + ii], (unsigned long long)slo2);
-            atomicAdd((unsigned long long *)&base[4 * B + ii], 1ull);
-            atomicMax((unsigned long long *)&base[5 * B + ii], (unsigned long long)maxbits);
+            add_piece(&sum[ii], P.s0);
+            add_piece(&sum[B + ii], P.s1);
+            add_piece(&sq[ii], P.q0);
+            add_piece(&sq[B + ii], P.q1);
+            add_piece(&sq[2 * B + ii], P.q2);
+            atomicAdd((unsigned long long *)&base[(OCTMPS_TALLY_CLASS_WORDS - 2) * B + ii], 1ull);
+            atomicMax((unsigned long long *)&base[(OCTMPS_TALLY_CLASS_WORDS - 1) * B + ii],
+                      (unsigned long long)maxbits);
         }
     }
 }
@@ -791,6 +780,12 @@ __global__ void probe_rng_kernel(const uint32_t *mult, uint32_t K, uint64_t seed
     }
 }
 
+// One detection per lane into scan slot 0's tally (the detector tally on chosen inputs).
+__global__ void probe_tally_kernel(KArgs A, DevSet S, int n, const float *op, const float *maxz, const float *c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tally_detection(A, S, 0, op[i], maxz[i], c[i]);
+}
+
 __global__ void probe_fresnel_kernel(int n, const float *ni, const float *nt, const float *ca1, float *r) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) r[i] = fresnel_f(ni[i], nt[i], ca1[i]);
@@ -876,8 +871,8 @@ __global__ void fold_tallies_kernel(uint64_t *p, uint64_t n_words, uint64_t word
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t q = i % words;  // word within one A-scan's tally
-        const uint64_t r = q >= 4 ? (q - 4) % (7 * bins) : 0;
-        const bool is_max = q >= 4 && r >= 5 * bins && r < 6 * bins;
+        const uint64_t r = q >= 4 ? (q - 4) % (OCTMPS_TALLY_CLASS_WORDS * bins) : 0;
+        const bool is_max = q >= 4 && r >= (OCTMPS_TALLY_CLASS_WORDS - 1) * bins;
         uint64_t acc = p[i];
         for (uint32_t c = 1; c < copies; ++c) {
             const uint64_t v = p[i + c * stride];
@@ -890,7 +885,7 @@ __global__ void fold_tallies_kernel(uint64_t *p, uint64_t n_words, uint64_t word
 hipError_t launch_fold_tallies(uint64_t *p, uint64_t n_scans, int bins, uint32_t copies, uint64_t stride,
                                hipStream_t stream) {
     if (copies <= 1 || n_scans == 0) return hipSuccess;
-    const uint64_t words = 4 + 14 * (uint64_t)bins;  // octmps_tally_words
+    const uint64_t words = 4 + 2 * OCTMPS_TALLY_CLASS_WORDS * (uint64_t)bins;  // octmps_tally_words
     const uint64_t n = n_scans * words;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
@@ -917,6 +912,12 @@ hipError_t launch_probe_rng(const uint32_t *mult, uint32_t K, uint64_t seed, uin
 hipError_t launch_probe_fresnel(int n, const float *ni, const float *nt, const float *ca1, float *r,
                                 hipStream_t stream) {
     hipLaunchKernelGGL(probe_fresnel_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, ni, nt, ca1, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_tally(const KArgs &a, const DevSet &S, int n, const float *op, const float *maxz,
+                              const float *c, hipStream_t stream) {
+    hipLaunchKernelGGL(probe_tally_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, S, n, op, maxz, c);
     return hipGetLastError();
 }
 

@@ -86,7 +86,7 @@ class RunStats(C.Structure):
         ("photons", C.c_uint64), ("photon_steps", C.c_uint64), ("killed", C.c_uint64),
         ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
         ("launches", C.c_uint32), ("block_size", C.c_uint32), ("grid_size", C.c_uint32),
-        ("refill_quantum", C.c_uint32), ("wave_iterations", C.c_uint64),
+        ("refill_quantum", C.c_uint32), ("wave_iterations", C.c_uint64), ("bad_contributions", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -173,6 +173,7 @@ SIGNATURES = [
     ("octmps_serve_poll", C.c_int, [_P, C.c_uint64, C.POINTER(RequestInfo)]),
     ("octmps_serve_wait", C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(RequestInfo)]),
     ("octmps_serve_cancel", C.c_int, [_P, C.c_uint64]),
+    ("octmps_serve_release", C.c_int, [_P, C.c_uint64]),
     ("octmps_serve_stats", C.c_int, [_P, _u64p, _u64p, C.POINTER(C.c_double), _u64p]),
     ("octmps_serve_close", C.c_int, [_P]),
     ("octmps_probe_rng", C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int32, C.c_int32,
@@ -181,6 +182,8 @@ SIGNATURES = [
                                        C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("octmps_probe_sampler", C.c_int, [_P, C.c_int32, C.c_float, C.c_int32, C.POINTER(C.c_float),
                                        C.POINTER(C.c_float)]),
+    ("octmps_probe_tally", C.c_int, [_P, C.POINTER(Params), C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float), _u64p, _u64p]),
     ("octmps_probe_copy", C.c_int, [_P, C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
     ("octmps_probe_gather", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
 ]
@@ -346,24 +349,52 @@ def tally_words(bins):
     return int(lib.octmps_tally_words(bins))
 
 
+TALLY_SUM_DIGITS, TALLY_SQ_DIGITS, TALLY_CLASS_WORDS = 9, 18, 29
+TALLY_SUM_EXP, TALLY_SQ_EXP = -160, -320
+
+
+def _digits_value(blk, exp0):
+    """sum_k blk[k] * 2^(32k + exp0), accumulated in extended precision (exact for the
+    statistics the tests need; octmps_tally_finalize converts exactly)."""
+    acc = np.zeros(blk.shape[1], np.longdouble)
+    for k in range(blk.shape[0] - 1, -1, -1):
+        acc += np.ldexp(blk[k].astype(np.longdouble), 32 * k + exp0)
+    return acc.astype(np.float64)
+
+
 def decode_tally(t, bins):
     """Integer tally -> dict of float64 arrays [class, bin] (class 0 = Class I)."""
     t = np.asarray(t, dtype=np.uint64)
-    out = {"num_filtered": int(t[0]), "num_class1": int(t[1]), "num_class2": int(t[2])}
+    out = {"num_filtered": int(t[0]), "num_class1": int(t[1]), "num_class2": int(t[2]), "bad": int(t[3])}
     s, sq, cnt, mx = [], [], [], []
     for c in range(2):
-        base = 4 + c * 7 * bins
-        blk = t[base:base + 7 * bins].reshape(7, bins)
-        # hi words in units of 2^-32, lo words in units of 2^-64, SumSq's lo2 in 2^-96 (include/octmps.h)
-        s.append(blk[1].astype(np.float64) * 2.0 ** -32 + blk[0].astype(np.float64) * 2.0 ** -64)
-        sq.append(blk[3].astype(np.float64) * 2.0 ** -32 + blk[2].astype(np.float64) * 2.0 ** -64
-                  + blk[6].astype(np.float64) * 2.0 ** -96)
-        cnt.append(blk[4].astype(np.float64))
-        mx.append(blk[5].view(np.float64).copy())
+        base = 4 + c * TALLY_CLASS_WORDS * bins
+        blk = t[base:base + TALLY_CLASS_WORDS * bins].reshape(TALLY_CLASS_WORDS, bins)
+        s.append(_digits_value(blk[:TALLY_SUM_DIGITS], TALLY_SUM_EXP))
+        sq.append(_digits_value(blk[TALLY_SUM_DIGITS:TALLY_SUM_DIGITS + TALLY_SQ_DIGITS], TALLY_SQ_EXP))
+        cnt.append(blk[TALLY_CLASS_WORDS - 2].astype(np.float64))
+        mx.append(blk[TALLY_CLASS_WORDS - 1].view(np.float64).copy())
     out["sum"] = np.array(s)
     out["sumsq"] = np.array(sq)
     out["count"] = np.array(cnt)
     out["max"] = np.array(mx)
+    return out
+
+
+def exact_tally_sums(t, bins):
+    """Exact Sum and SumSq of every bin as Python Fractions: [class][bin] -> (Sum, SumSq)."""
+    from fractions import Fraction
+    t = np.asarray(t, dtype=np.uint64)
+    out = []
+    for c in range(2):
+        base = 4 + c * TALLY_CLASS_WORDS * bins
+        blk = t[base:base + TALLY_CLASS_WORDS * bins].reshape(TALLY_CLASS_WORDS, bins)
+        row = []
+        for b in range(bins):
+            s = sum(int(blk[k, b]) << (32 * k) for k in range(TALLY_SUM_DIGITS))
+            q = sum(int(blk[TALLY_SUM_DIGITS + k, b]) << (32 * k) for k in range(TALLY_SQ_DIGITS))
+            row.append((Fraction(s, 2 ** -TALLY_SUM_EXP), Fraction(q, 2 ** -TALLY_SQ_EXP)))
+        out.append(row)
     return out
 
 
@@ -394,6 +425,9 @@ class Context:
         mid = C.c_int32(-1)
         check(lib.octmps_ctx_upload_mesh(self.h, mesh.h, C.byref(mid)), "upload_mesh")
         return mid.value
+
+    def release(self, mesh_id):
+        check(lib.octmps_ctx_release_mesh(self.h, mesh_id), "release_mesh")
 
     def run_ascan(self, mesh_id, params, scan, seed, opts=None):
         bins = params.num_depth_bins
@@ -439,6 +473,15 @@ class Context:
         r = np.empty(len(ni), np.float32)
         check(lib.octmps_probe_fresnel(self.h, len(ni), _fp(ni), _fp(nt), _fp(ca1), _fp(r)), "probe_fresnel")
         return r
+
+    def probe_tally(self, params, op, maxz, contrib):
+        """Device detector tally of the given detections into one zeroed tally: (tally, bad)."""
+        op, maxz, contrib = (np.ascontiguousarray(v, np.float32) for v in (op, maxz, contrib))
+        t = np.zeros(tally_words(params.num_depth_bins), np.uint64)
+        bad = C.c_uint64(0)
+        check(lib.octmps_probe_tally(self.h, C.byref(params), len(op), _fp(op), _fp(maxz), _fp(contrib), _up(t),
+                                     C.byref(bad)), "probe_tally")
+        return t, bad.value
 
     def probe_copy(self, nbytes, repeats=3):
         ms = C.c_double(0)
@@ -497,6 +540,11 @@ class Server:
 
     def cancel(self, rid):
         check(lib.octmps_serve_cancel(self.h, rid), "serve_cancel")
+
+    def release(self, rid):
+        """Forget a finished request (octmps_serve_release) and drop its output buffer."""
+        check(lib.octmps_serve_release(self.h, rid), "serve_release")
+        self._keep.pop(rid, None)
 
     def stats(self):
         ph, st, sl = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -94,8 +94,9 @@ def bump(cells=(128, 128, 64), n_scans=1024, photons=1_000_000, pitch=0.0055 * M
     return mesh, params, scans, 1
 
 
-def served_stream(rate, duration_s, seed=11, n_meshes=4):
-    """Poisson request trace: (arrival_s, mesh index, n_scans, per-A-scan budgets)."""
+def served_stream(rate, duration_s, seed=11, n_meshes=4, lo=1e4, hi=1e6):
+    """Poisson request trace: (arrival_s, mesh index, n_scans, per-A-scan budgets), budgets
+    log-uniform in [lo, hi] (the served stream's [1e4, 1e6] by default)."""
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     while True:
@@ -104,6 +105,6 @@ def served_stream(rate, duration_s, seed=11, n_meshes=4):
             break
         mesh = int(rng.integers(0, n_meshes))
         n = int(rng.integers(1, 257))
-        budgets = np.rint(np.exp(rng.uniform(np.log(1e4), np.log(1e6), n))).astype(np.int64)
+        budgets = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), n))).astype(np.int64)
         out.append((t, mesh, n, budgets))
     return out

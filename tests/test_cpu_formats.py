@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import octmps as O
+from tally_util import make_tally
 import oracle_py as OR
 
 INPUT = "/root/reference/input"
@@ -97,24 +98,6 @@ def test_mesh_serialisation_like_reference(case):
     assert mine == root
 
 
-def _tally_from(sums, sqs, maxs, cnts, nf):
-    """Integer tally whose sums are exactly the given doubles (multiples of 2^-30): the high
-    word counts 2^-32 units, the low word 2^-64 units."""
-    B = sums.shape[1]
-    t = np.zeros(4 + 14 * B, np.uint64)
-    t[0:3] = nf
-    for c in range(2):
-        base = 4 + c * 7 * B
-        for arr, off in ((sums[c], 0), (sqs[c], 2)):
-            k = np.rint(arr * 2.0 ** 30).astype(np.uint64)
-            # value = k * 2^-30 = (4k) * 2^-32: all in the high word, low word 0
-            t[base + off * B:base + (off + 1) * B] = 0
-            t[base + (off + 1) * B:base + (off + 2) * B] = k << np.uint64(2)
-        t[base + 4 * B:base + 5 * B] = cnts[c]
-        t[base + 5 * B:base + 6 * B] = maxs[c].view(np.uint64)
-    return t
-
-
 @needs_ref
 def test_writer_byte_identical_to_reference(tmp_path):
     opt, mci, _ = (os.path.join(INPUT, f) if f else None for f in CASES["Sphere"])
@@ -125,10 +108,11 @@ def test_writer_byte_identical_to_reference(tmp_path):
     k = rng.integers(0, 2 ** 36, size=(2, B)) * (rng.random((2, B)) < 0.7)
     sums = k.astype(np.float64) * 2.0 ** -30
     sqs = np.rint(sums * sums * 0.3 * 2 ** 30) * 2.0 ** -30
-    maxs = np.where(rng.random((2, B)) < 0.5, sums * 0.5, sums * 0.001)
+    # Max is an FP32 contribution on the device, so the test's maxima are FP32 values.
+    maxs = np.where(rng.random((2, B)) < 0.5, sums * 0.5, sums * 0.001).astype(np.float32).astype(np.float64)
     cnts = rng.integers(0, 1000, size=(2, B)).astype(np.uint64)
     nf = [12345, 678, 90]
-    t = _tally_from(sums, sqs, maxs, cnts, nf)
+    t = make_tally(sums, sqs, maxs, cnts, nf)
     dec = O.decode_tally(t, B)
     assert np.array_equal(dec["sum"], sums) and np.array_equal(dec["sumsq"], sqs)
     mine, theirs = tmp_path / "mine", tmp_path / "ref"

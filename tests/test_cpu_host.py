@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import octmps as O
+from tally_util import make_tally
 from octmps import configs
 
 
@@ -102,29 +103,18 @@ def test_input_errors(tmp_path):
 def test_tally_finalize_formula():
     B = 8
     rng = np.random.default_rng(0)
-    t = np.zeros(O.tally_words(B), np.uint64)
     n = 1000
-    vals = {}
-    for c in range(2):
-        base = 4 + c * 7 * B
-        s = rng.integers(0, 2 ** 40, B).astype(np.uint64)
-        q = (s.astype(np.float64) ** 2 / 2 ** 20 / n * rng.uniform(1, 3, B)).astype(np.uint64)
-        # value s * 2^-20: high word (units 2^-32) = s * 2^12 - (low part), low word (2^-64 units)
-        t[base + B:base + 2 * B] = (s >> np.uint64(4)) << np.uint64(16)
-        t[base:base + B] = (s & np.uint64(15)) << np.uint64(44)
-        t[base + 3 * B:base + 4 * B] = (q >> np.uint64(4)) << np.uint64(16)
-        t[base + 2 * B:base + 3 * B] = (q & np.uint64(15)) << np.uint64(44)
-        mx = (s.astype(np.float64) * 2.0 ** -20) * rng.uniform(0, 0.03, B)
-        t[base + 5 * B:base + 6 * B] = mx.view(np.uint64)
-        vals[c] = (s.astype(np.float64) * 2.0 ** -20, q.astype(np.float64) * 2.0 ** -20, mx)
+    s = rng.integers(0, 2 ** 40, (2, B)).astype(np.float64) * 2.0 ** -20
+    q = np.floor(s ** 2 / n * rng.uniform(1, 3, (2, B)) * 2 ** 20) * 2.0 ** -20
+    mx = (s * rng.uniform(0, 0.03, (2, B))).astype(np.float32).astype(np.float64)
+    t = make_tally(s, q, mx, np.zeros((2, B), np.uint64), [0, 0, 0])
     out = O.finalize(t, B, n, 0.01)
     for c in range(2):
-        s, q, mx = vals[c]
-        mean = s / n
-        se = np.sqrt((q / n - mean ** 2) / (n - 1))
+        mean = s[c] / n
+        se = np.sqrt((q[c] / n - mean ** 2) / (n - 1))
         assert np.allclose(out[0, c, 0], mean, rtol=1e-15) and np.allclose(out[0, c, 1], se, rtol=1e-12, equal_nan=True)
-        cut = mx > 0.01 * s
-        s2, q2 = np.where(cut, s - mx, s), np.where(cut, q - mx * mx, q)
+        cut = mx[c] > 0.01 * s[c]
+        s2, q2 = np.where(cut, s[c] - mx[c], s[c]), np.where(cut, q[c] - mx[c] * mx[c], q[c])
         assert np.allclose(out[1, c, 0], s2 / n, rtol=1e-15)
         assert np.allclose(out[1, c, 1], np.sqrt((q2 / n - (s2 / n) ** 2) / (n - 1)), rtol=1e-12, equal_nan=True)
 
@@ -165,3 +155,61 @@ def test_benchmark_workloads_match_committed_traces():
         for (t, m, n, bud), r in zip(got, s["requests"]):
             assert t == r["t"] and m == r["mesh"] and n == r["n_scans"] and [int(x) for x in bud] == r["budgets"]
             assert 0 <= m < 4 and 1 <= n <= 256 and len(bud) == n
+
+
+def test_single_contribution_bin_has_zero_filtered_se():
+    # A bin holding one contribution c: the filtered outputs drop it (c > MaxRelContrib * c),
+    # leaving Sum - Max = 0 and SumSq - Max^2 = 0 exactly, so the filtered SE is 0 (the
+    # reference's FP64 arithmetic gives 0 there too), never NaN. Contributions span the FP32
+    # range, subnormals included.
+    cs = np.array([1.0, 0.3, 1e-16, 3.6e-15, 5e-20, 2e-40, 1e5, 4e9, 1e30], np.float32).astype(np.float64)
+    B = len(cs)
+    z = np.zeros((2, B))
+    sums = np.stack([cs, z[0]])
+    t = make_tally(sums, sums * sums, sums.copy(), np.stack([np.ones(B, np.uint64), np.zeros(B, np.uint64)]),
+                   [B, B, 0])
+    n = 1000
+    out = O.finalize(t, B, n, 0.01)
+    assert np.array_equal(out[1, 0, 0], np.zeros(B)) and np.array_equal(out[1, 0, 1], np.zeros(B))
+    assert np.all(np.isfinite(out[0, 0, 1])) and np.all(out[0, 0, 1] > 0)
+    assert np.allclose(out[0, 0, 0], cs / n, rtol=1e-15, atol=0)
+    # SE of one contribution c in N photons: sqrt((c^2/N - c^2/N^2) / (N-1)) = c / N.
+    assert np.allclose(out[0, 0, 1], cs / n, rtol=1e-12, atol=0)
+
+
+def test_finalize_sums_over_the_whole_fp32_range():
+    # Sums of many FP32 contributions from 1e-40 to 1e38 are held exactly; the FP64 mean is
+    # the correctly rounded exact sum divided by N (to an ulp), whatever the spread.
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    B = 6
+    cs = [np.float32(10.0) ** rng.uniform(lo, hi, 40).astype(np.float32)
+          for lo, hi in ((-40, -30), (-20, -10), (-5, 0), (0, 5), (10, 20), (30, 38))]
+    exact_s = [sum(Fraction(float(c)) for c in row) for row in cs]
+    exact_q = [sum(Fraction(float(c)) ** 2 for c in row) for row in cs]
+    maxs = np.array([max(float(c) for c in row) for row in cs])
+    sums = np.array([[float(s) for s in exact_s], [0.0] * B])
+    # make_tally needs exactly representable sums: encode the exact Fractions directly.
+    t = make_tally(np.zeros((2, B)), np.zeros((2, B)), np.stack([maxs, np.zeros(B)]),
+                   np.zeros((2, B), np.uint64), [0, 0, 0])
+    W = O.TALLY_CLASS_WORDS
+    for b in range(B):
+        S = exact_s[b] * 2 ** -O.TALLY_SUM_EXP
+        Q = exact_q[b] * 2 ** -O.TALLY_SQ_EXP
+        assert S.denominator == 1 and Q.denominator == 1
+        for k in range(O.TALLY_SUM_DIGITS):
+            t[4 + k * B + b] = (int(S) >> (32 * k)) & 0xFFFFFFFF
+        for k in range(O.TALLY_SQ_DIGITS):
+            t[4 + (O.TALLY_SUM_DIGITS + k) * B + b] = (int(Q) >> (32 * k)) & 0xFFFFFFFF
+    n = 10 ** 6
+    out = O.finalize(t, B, n, 0.01)
+    for b in range(B):
+        mean = float(exact_s[b] / n)
+        assert abs(out[0, 0, 0, b] - mean) <= 2 * np.spacing(mean)
+        var = float((exact_q[b] / n - (exact_s[b] / n) ** 2) / (n - 1))
+        assert abs(out[0, 0, 1, b] - np.sqrt(var)) <= 1e-12 * np.sqrt(var)
+        # filtered: the largest contribution removed exactly (it exceeds 1 % of these sums)
+        if maxs[b] > 0.01 * float(exact_s[b]):
+            fs = exact_s[b] - Fraction(maxs[b])
+            assert abs(out[1, 0, 0, b] - float(fs / n)) <= 2 * np.spacing(float(fs / n)) + 1e-300
+    assert np.allclose(sums[0], [float(s) for s in exact_s])
