@@ -328,7 +328,15 @@ void scheduler(octmps_server *s) {
         A.scatter_q = 32;
         A.slice_mode = 1;
         A.restore = s->have_saved ? 1u : 0u;
-        A.exit_on_drain = total > 0 ? 1u : 0u;
+        // A slice whose table lists every unclaimed id runs its full length (waves leave once
+        // their photons are done and the ids are gone); ending it as soon as the ids are
+        // handed out would save and restore every live photon each time a request arrives,
+        // which under a steady stream of small requests left most slices a few iterations
+        // long. Only a table the budget cut short ends on drain, so that the next slice
+        // lists the rest.
+        uint64_t unclaimed = 0;
+        for (const OpenScan &o : os) unclaimed += o.rem;
+        A.exit_on_drain = (total > 0 && total < unclaimed) ? 1u : 0u;
         A.sets_in_lds = sets_in_lds ? 1u : 0u;
         A.set_slot = (const uint8_t *)s->set_map.p;
         A.lds_set_ids = (const int32_t *)((const uint8_t *)s->set_map.p + sizeof s->set_slot_h);
