@@ -276,6 +276,19 @@ octmps_status octmps_run_bscan(octmps_ctx *ctx, int32_t mesh_id, const octmps_pa
                                int32_t n_scans, const octmps_ascan *scans, uint64_t seed,
                                const octmps_batch_opts *opts, uint64_t *tallies_out,
                                octmps_run_stats *stats);
+/* octmps_run_bscan that allocates no device memory: every device buffer of the run comes
+ * from one file-scope device array of 64 MiB per GPU (held until the process ends), so it
+ * calls no hipMalloc/hipFree. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid allocation). Same
+ * arguments and the same tallies, bit for bit, as octmps_run_bscan; always launched eagerly
+ * (opts->use_graph is ignored). A run needing more than 64 MiB of scratch (per-lane state
+ * ~19 MB plus n_scans * tally_copies * octmps_tally_words(B) * 8 bytes, e.g. 128 A-scans at
+ * 512 bins with one tally copy: ~50 MB) returns OCTMPS_E_RANGE. Calls on one GPU share the
+ * array: calls that overlap on one GPU (two contexts on the same device) race. */
+octmps_status octmps_run_bscan_noalloc(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p,
+                                       int32_t n_scans, const octmps_ascan *scans, uint64_t seed,
+                                       const octmps_batch_opts *opts, uint64_t *tallies_out,
+                                       octmps_run_stats *stats);
 /* octmps_run_bscan that also writes one 16-byte record per photon, in launch order
  * (photon ids of scan 0 first): {u32 steps, u32 detections, f32 summed contribution,
  * u32 killed}. Diagnostic path used to pair GPU photons with the CPU oracle's; no

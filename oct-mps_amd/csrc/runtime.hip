@@ -157,10 +157,33 @@ void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &gr
 }
 
 // Runs a list of A-scans to completion in one persistent launch (drain mode).
+// Device buffers of one run: from the context's growable work buffers, or carved from a
+// caller-provided arena (the allocation-free entry point) when `arena` is set.
+struct RunBuffers {
+    void *regions, *sets, *slots, *seg_start, *seg_slot, *seg_poff, *tallies, *ctrl, *slot_done, *plog, *cont;
+};
+
+// Carves 256-byte-aligned pieces off an arena in order; `ok` turns false once a piece does
+// not fit.
+struct ArenaCarver {
+    char *base;
+    size_t cap, used = 0;
+    bool ok = true;
+    void *take(size_t n) {
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (at + n > cap) {
+            ok = false;
+            return nullptr;
+        }
+        used = at + n;
+        return base + at;
+    }
+};
+
 octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p, int32_t n,
                         const octmps_ascan *scans, uint64_t seed, const octmps_batch_opts *opts,
                         uint64_t *tallies_out, octmps_run_stats *stats, PhotonLogRec *plog_out,
-                        uint64_t plog_n) {
+                        uint64_t plog_n, void *arena, size_t arena_bytes) {
     double t0 = now_ms();
     if (!ctx || !p || (n > 0 && !scans) || !tallies_out) return fail(OCTMPS_E_INVALID, "null argument");
     if (n <= 0) return fail(OCTMPS_E_INVALID, "no A-scans");
@@ -207,44 +230,64 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     grid_for(ctx, block, opts ? opts->blocks_per_cu : 0, false, grid);
     hipError_t e;
     hipStream_t sm = ctx->stream;
-    WorkBuffers &W = ctx->work;
-    if ((e = W.regions.ensure(R.size() * sizeof(DevRegion))) != hipSuccess ||
-        (e = W.sets.ensure(sizeof(DevSet))) != hipSuccess ||
-        (e = W.slots.ensure(slots.size() * sizeof(DevScan))) != hipSuccess ||
-        (e = W.seg_start.ensure(seg_start.size() * 8)) != hipSuccess ||
-        (e = W.seg_slot.ensure(seg_slot.size() * 4)) != hipSuccess ||
-        (e = W.seg_poff.ensure(seg_poff.size() * 8)) != hipSuccess ||
-        (e = W.tallies.ensure((size_t)copies * n * words * 8)) != hipSuccess ||
-        (e = W.ctrl.ensure(CTRL_WORDS * 8)) != hipSuccess ||
-        (e = W.slot_done.ensure((size_t)n * 8)) != hipSuccess ||
-        (e = W.cont.ensure((size_t)grid * block * sizeof(ContRec))) != hipSuccess)
-        return hip_fail(e, "device allocation");
-    if (plog_out && (e = W.plog.ensure(total * sizeof(PhotonLogRec))) != hipSuccess)
-        return hip_fail(e, "photon log allocation");
-    if ((e = hipMemcpyAsync(W.regions.p, R.data(), R.size() * sizeof(DevRegion), hipMemcpyHostToDevice, sm)) ||
-        (e = hipMemcpyAsync(W.sets.p, &S, sizeof S, hipMemcpyHostToDevice, sm)) ||
-        (e = hipMemcpyAsync(W.slots.p, slots.data(), slots.size() * sizeof(DevScan), hipMemcpyHostToDevice, sm)) ||
-        (e = hipMemcpyAsync(W.seg_start.p, seg_start.data(), seg_start.size() * 8, hipMemcpyHostToDevice, sm)) ||
-        (e = hipMemcpyAsync(W.seg_slot.p, seg_slot.data(), seg_slot.size() * 4, hipMemcpyHostToDevice, sm)) ||
-        (e = hipMemcpyAsync(W.seg_poff.p, seg_poff.data(), seg_poff.size() * 8, hipMemcpyHostToDevice, sm)))
+    RunBuffers D;
+    const size_t sz_regions = R.size() * sizeof(DevRegion), sz_slots = slots.size() * sizeof(DevScan);
+    const size_t sz_tallies = (size_t)copies * n * words * 8, sz_cont = (size_t)grid * block * sizeof(ContRec);
+    if (arena) {
+        ArenaCarver ac{(char *)arena, arena_bytes};
+        D.regions = ac.take(sz_regions);
+        D.sets = ac.take(sizeof(DevSet));
+        D.slots = ac.take(sz_slots);
+        D.seg_start = ac.take(seg_start.size() * 8);
+        D.seg_slot = ac.take(seg_slot.size() * 4);
+        D.seg_poff = ac.take(seg_poff.size() * 8);
+        D.tallies = ac.take(sz_tallies);
+        D.ctrl = ac.take(CTRL_WORDS * 8);
+        D.slot_done = ac.take((size_t)n * 8);
+        D.cont = ac.take(sz_cont);
+        D.plog = nullptr;
+        if (!ac.ok)
+            return fail(OCTMPS_E_RANGE, "run needs " + std::to_string(ac.used) + "+ bytes of device scratch; the " +
+                                            "allocation-free entry point has " + std::to_string(arena_bytes));
+    } else {
+        WorkBuffers &W = ctx->work;
+        if ((e = W.regions.ensure(sz_regions)) != hipSuccess || (e = W.sets.ensure(sizeof(DevSet))) != hipSuccess ||
+            (e = W.slots.ensure(sz_slots)) != hipSuccess ||
+            (e = W.seg_start.ensure(seg_start.size() * 8)) != hipSuccess ||
+            (e = W.seg_slot.ensure(seg_slot.size() * 4)) != hipSuccess ||
+            (e = W.seg_poff.ensure(seg_poff.size() * 8)) != hipSuccess ||
+            (e = W.tallies.ensure(sz_tallies)) != hipSuccess || (e = W.ctrl.ensure(CTRL_WORDS * 8)) != hipSuccess ||
+            (e = W.slot_done.ensure((size_t)n * 8)) != hipSuccess || (e = W.cont.ensure(sz_cont)) != hipSuccess)
+            return hip_fail(e, "device allocation");
+        if (plog_out && (e = W.plog.ensure(total * sizeof(PhotonLogRec))) != hipSuccess)
+            return hip_fail(e, "photon log allocation");
+        D = {W.regions.p, W.sets.p, W.slots.p, W.seg_start.p, W.seg_slot.p, W.seg_poff.p,
+             W.tallies.p, W.ctrl.p, W.slot_done.p, W.plog.p, W.cont.p};
+    }
+    if ((e = hipMemcpyAsync(D.regions, R.data(), sz_regions, hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(D.sets, &S, sizeof S, hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(D.slots, slots.data(), sz_slots, hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(D.seg_start, seg_start.data(), seg_start.size() * 8, hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(D.seg_slot, seg_slot.data(), seg_slot.size() * 4, hipMemcpyHostToDevice, sm)) ||
+        (e = hipMemcpyAsync(D.seg_poff, seg_poff.data(), seg_poff.size() * 8, hipMemcpyHostToDevice, sm)))
         return hip_fail(e, "upload");
     KArgs A;
     std::memset(&A, 0, sizeof A);
     A.tets = ctx->pool_tets;
     A.tets_bytes = (uint32_t)(ctx->pool_used * (int64_t)sizeof(TetRec));
     A.skip_entry = skip_opt == 1 || (skip_opt == 0 && entry_skip_fits(me.n_tets));
-    A.regions = (const DevRegion *)W.regions.p;
-    A.sets = (const DevSet *)W.sets.p;
-    A.slots = (const DevScan *)W.slots.p;
-    A.seg_start = (const uint64_t *)W.seg_start.p;
-    A.seg_slot = (const uint32_t *)W.seg_slot.p;
-    A.seg_poff = (const uint64_t *)W.seg_poff.p;
+    A.regions = (const DevRegion *)D.regions;
+    A.sets = (const DevSet *)D.sets;
+    A.slots = (const DevScan *)D.slots;
+    A.seg_start = (const uint64_t *)D.seg_start;
+    A.seg_slot = (const uint32_t *)D.seg_slot;
+    A.seg_poff = (const uint64_t *)D.seg_poff;
     A.mult = ctx->d_mult;
-    A.tallies = (uint64_t *)W.tallies.p;
-    A.ctrl = (uint64_t *)W.ctrl.p;
-    A.slot_done = (uint64_t *)W.slot_done.p;
-    A.plog = plog_out ? (PhotonLogRec *)W.plog.p : nullptr;
-    A.cont = (ContRec *)W.cont.p;
+    A.tallies = (uint64_t *)D.tallies;
+    A.ctrl = (uint64_t *)D.ctrl;
+    A.slot_done = (uint64_t *)D.slot_done;
+    A.plog = plog_out ? (PhotonLogRec *)D.plog : nullptr;
+    A.cont = (ContRec *)D.cont;
     A.id_end = total;
     A.max_steps = (opts && opts->max_steps_per_photon) ? opts->max_steps_per_photon : (1ull << 22);
     A.n_seg = (int32_t)seg_slot.size();
@@ -253,7 +296,8 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     A.scatter_q = (uint32_t)((opts && opts->scatter_quantum > 0) ? opts->scatter_quantum : default_scatter(ctx));
     A.tally_copies = (uint32_t)copies;
     A.copy_stride = (uint64_t)n * words;
-    const bool use_graph = opts && opts->use_graph && !plog_out;
+    // The allocation-free path never captures a graph (instantiation allocates).
+    const bool use_graph = opts && opts->use_graph && !plog_out && !arena;
     // Graph mode records the events around the graph launch (the fill kernels add a few us);
     // eager mode records them around the transport kernel alone.
     auto enqueue = [&](hipStream_t s2, bool timed) -> hipError_t {
@@ -352,6 +396,10 @@ extern "C" octmps_status octmps_ctx_create(int32_t device, octmps_ctx **out) {
         (e = hipMemcpy(c->d_mult, t->mult.data(), t->mult.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) {
         octmps_ctx_destroy(c);
         return hip_fail(e, "context setup");
+    }
+    if ((st = scratch_init(c)) != OCTMPS_OK) {
+        octmps_ctx_destroy(c);
+        return st;
     }
     *out = c;
     return OCTMPS_OK;
@@ -475,13 +523,13 @@ extern "C" octmps_status octmps_ctx_memory(octmps_ctx *c, uint64_t *used, uint64
 extern "C" octmps_status octmps_run_ascan(octmps_ctx *c, int32_t mesh_id, const octmps_params *p,
                                           const octmps_ascan *scan, uint64_t seed, const octmps_batch_opts *opts,
                                           uint64_t *tally_out, octmps_run_stats *stats) {
-    return run_scans(c, mesh_id, p, 1, scan, seed, opts, tally_out, stats, nullptr, 0);
+    return run_scans(c, mesh_id, p, 1, scan, seed, opts, tally_out, stats, nullptr, 0, nullptr, 0);
 }
 
 extern "C" octmps_status octmps_run_bscan(octmps_ctx *c, int32_t mesh_id, const octmps_params *p, int32_t n,
                                           const octmps_ascan *scans, uint64_t seed, const octmps_batch_opts *opts,
                                           uint64_t *tallies_out, octmps_run_stats *stats) {
-    return run_scans(c, mesh_id, p, n, scans, seed, opts, tallies_out, stats, nullptr, 0);
+    return run_scans(c, mesh_id, p, n, scans, seed, opts, tallies_out, stats, nullptr, 0, nullptr, 0);
 }
 
 // Runs like octmps_run_bscan and also returns per-photon summaries (steps, detections,
@@ -492,7 +540,7 @@ extern "C" octmps_status octmps_run_bscan_logged(octmps_ctx *c, int32_t mesh_id,
                                                  const octmps_batch_opts *opts, uint64_t *tallies_out,
                                                  octmps_run_stats *stats, void *photon_log, uint64_t log_records) {
     return run_scans(c, mesh_id, p, n, scans, seed, opts, tallies_out, stats, (PhotonLogRec *)photon_log,
-                     log_records);
+                     log_records, nullptr, 0);
 }
 
 extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const octmps_params *p, int32_t n,
@@ -519,7 +567,7 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     win.use_graph = 0;
     auto trial = [&](octmps_batch_opts o) -> octmps_status {
         octmps_run_stats st;
-        octmps_status s = run_scans(c, mesh_id, p, n, cs.data(), seed, &o, t.data(), &st, nullptr, 0);
+        octmps_status s = run_scans(c, mesh_id, p, n, cs.data(), seed, &o, t.data(), &st, nullptr, 0, nullptr, 0);
         if (s != OCTMPS_OK) return s;
         double r = st.kernel_ms > 0 ? st.photons / (st.kernel_ms * 1e-3) : 0;
         if (r > best) {

@@ -74,12 +74,24 @@ struct octmps_ctx {
     octmps::WorkBuffers work;
     octmps::GraphCache graph;
     int tuned_block = 0, tuned_q = 0, tuned_sq = 0, tuned_copies = 0;
+    // Device address and size of the allocation-free path's scratch array (noalloc.hip).
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
 };
 
 namespace octmps {
 octmps_status make_scan(octmps_ctx *ctx, const MeshEntry &me, const octmps_params *p, const octmps_ascan &s,
                         uint64_t seed, int set, uint64_t tally_off, DevScan &d);
 void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid);
+// Runs A-scans to completion in one persistent launch; device buffers come from the context's
+// work buffers, or from `arena` (arena_bytes) when it is set (runtime.hip).
+octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p, int32_t n,
+                        const octmps_ascan *scans, uint64_t seed, const octmps_batch_opts *opts,
+                        uint64_t *tallies_out, octmps_run_stats *stats, PhotonLogRec *plog_out,
+                        uint64_t plog_n, void *arena, size_t arena_bytes);
+// Looks up the device address of the allocation-free path's scratch array (noalloc.hip);
+// called by octmps_ctx_create with the context's device current.
+octmps_status scratch_init(octmps_ctx *ctx);
 }  // namespace octmps
 
 // The entry-face skip pays when tet records come from cache: it is on by default for meshes

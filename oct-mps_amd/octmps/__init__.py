@@ -164,6 +164,8 @@ SIGNATURES = [
                                    C.POINTER(BatchOpts), _u64p, C.POINTER(RunStats)]),
     ("octmps_run_bscan", C.c_int, [_P, C.c_int32, C.POINTER(Params), C.c_int32, C.POINTER(AScan), C.c_uint64,
                                    C.POINTER(BatchOpts), _u64p, C.POINTER(RunStats)]),
+    ("octmps_run_bscan_noalloc", C.c_int, [_P, C.c_int32, C.POINTER(Params), C.c_int32, C.POINTER(AScan), C.c_uint64,
+                                           C.POINTER(BatchOpts), _u64p, C.POINTER(RunStats)]),
     ("octmps_run_bscan_logged", C.c_int, [_P, C.c_int32, C.POINTER(Params), C.c_int32, C.POINTER(AScan), C.c_uint64,
                                           C.POINTER(BatchOpts), _u64p, C.POINTER(RunStats), _P, C.c_uint64]),
     ("octmps_autotune", C.c_int, [_P, C.c_int32, C.POINTER(Params), C.c_int32, C.POINTER(AScan), C.c_uint64,
