@@ -216,7 +216,18 @@ void octmps_ctx_destroy(octmps_ctx *ctx);
  * 2^25 - 1 tets (neighbour ids are packed with the neighbour's region), and regions must
  * be < 100 (OCTMPS_E_RANGE otherwise). */
 octmps_status octmps_ctx_upload_mesh(octmps_ctx *ctx, const octmps_mesh *m, int32_t *mesh_id);
+/* Register a mesh without uploading it (no counterpart in the reference): its 128-byte tet
+ * records are kept in host memory and made resident on demand - by a run on it, or by a
+ * request server when it admits a request on it - and a registered mesh that nothing uses
+ * may be evicted from the device again when a server needs the room (it is then uploaded
+ * again when next used). Results do not depend on where or how often it was uploaded. */
+octmps_status octmps_ctx_register_mesh(octmps_ctx *ctx, const octmps_mesh *m, int32_t *mesh_id);
+/* Release a mesh (src/main.cu:638's cudaFree): OCTMPS_E_BUSY while a submitted request or a
+ * run still uses it. */
 octmps_status octmps_ctx_release_mesh(octmps_ctx *ctx, int32_t mesh_id);
+/* Whether a mesh's records are on the device now, and their size in bytes. */
+octmps_status octmps_ctx_mesh_info(octmps_ctx *ctx, int32_t mesh_id, int32_t *resident,
+                                   uint64_t *device_bytes);
 /* Device bytes in use (meshes + work buffers) and HBM total. */
 octmps_status octmps_ctx_memory(octmps_ctx *ctx, uint64_t *used, uint64_t *total);
 
@@ -313,11 +324,15 @@ octmps_status octmps_autotune(octmps_ctx *ctx, int32_t mesh_id, const octmps_par
 typedef struct octmps_server octmps_server;
 
 typedef struct {
-    int32_t max_inflight_scans;  /* scan slots on the device (0 = 4096) */
+    int32_t max_inflight_scans;  /* scan slots on the device (0 = 32768) */
     int32_t block_size;          /* 0 = default */
     int32_t refill_quantum;      /* 0 = default */
     int32_t fail_request_index;  /* test hook: submit index to fail (-1 = none) */
-    uint64_t hbm_budget_bytes;   /* admission budget for tallies+state (0 = 1 GiB) */
+    uint64_t hbm_budget_bytes;   /* admission budget for the requests' HBM footprint: each
+                                    admitted request's tallies, scan slots and parameter set,
+                                    plus the device records of every mesh the server keeps
+                                    resident for them (0 = 3/4 of the device's memory). The
+                                    server's own per-lane state (~176 B per lane) is extra. */
     uint64_t slice_photons;      /* photon ids handed out per slice (0 = auto) */
     int32_t policy;              /* order of photon hand-out among requests in flight:
                                     0 = least remaining photons first, aged by waiting time
@@ -354,6 +369,10 @@ octmps_status octmps_serve_cancel(octmps_server *s, uint64_t req_id);
  * long-running server does not keep every request it has served. OCTMPS_E_BUSY while the
  * request is still running. No counterpart in the reference. */
 octmps_status octmps_serve_release(octmps_server *s, uint64_t req_id);
+/* Mesh uploads and evictions the server made since open, and the footprint bytes admitted
+ * now against its budget. */
+octmps_status octmps_serve_residency(octmps_server *s, uint64_t *uploads, uint64_t *evictions,
+                                     uint64_t *used_bytes, uint64_t *budget_bytes);
 /* Aggregate counters since open: photons, photon-steps, busy kernel ms, slices. */
 octmps_status octmps_serve_stats(octmps_server *s, uint64_t *photons, uint64_t *steps,
                                  double *kernel_ms, uint64_t *slices);

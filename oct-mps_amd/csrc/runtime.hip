@@ -188,11 +188,17 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     if (!ctx || !p || (n > 0 && !scans) || !tallies_out) return fail(OCTMPS_E_INVALID, "null argument");
     if (n <= 0) return fail(OCTMPS_E_INVALID, "no A-scans");
     std::lock_guard<std::mutex> lk(ctx->run_mu);
+    // The run counts as a user of the mesh (resident, not evicted or released) until it ends.
+    octmps_status ast = mesh_acquire_run(ctx, mesh_id);
+    if (ast != OCTMPS_OK) return ast;
+    struct RunRef {
+        octmps_ctx *c;
+        int32_t id;
+        ~RunRef() { mesh_release_run(c, id); }
+    } run_ref{ctx, mesh_id};
     std::shared_lock<std::shared_mutex> pk(ctx->pool_mu);
-    if (mesh_id < 0 || mesh_id >= (int)ctx->meshes.size() || !ctx->meshes[mesh_id].live)
-        return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
     if (hipError_t se = hipSetDevice(ctx->device); se != hipSuccess) return hip_fail(se, "set device");
-    const MeshEntry &me = ctx->meshes[mesh_id];
+    const MeshEntry &me = *ctx->meshes[mesh_id];
     DevSet S;
     std::vector<DevRegion> R(kMaxRegions);
     octmps_status st = make_set(p, me.max_region, 0, S, R.data());
@@ -274,7 +280,7 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     KArgs A;
     std::memset(&A, 0, sizeof A);
     A.tets = ctx->pool_tets;
-    A.tets_bytes = (uint32_t)(ctx->pool_used * (int64_t)sizeof(TetRec));
+    A.tets_bytes = (uint32_t)(ctx->pool_cap * (int64_t)sizeof(TetRec));
     A.skip_entry = skip_opt == 1 || (skip_opt == 0 && entry_skip_fits(me.n_tets));
     A.regions = (const DevRegion *)D.regions;
     A.sets = (const DevSet *)D.sets;
@@ -417,107 +423,6 @@ extern "C" void octmps_ctx_destroy(octmps_ctx *c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
-}
-
-extern "C" octmps_status octmps_ctx_upload_mesh(octmps_ctx *c, const octmps_mesh *mh, int32_t *mesh_id) {
-    if (!c || !mh || !mesh_id) return fail(OCTMPS_E_INVALID, "null argument");
-    const HostMesh &m = mh->m;
-    const int64_t T = m.ntets();
-    if (m.max_region >= kMaxRegions) return fail(OCTMPS_E_RANGE, "regions must be < 100");
-    std::unique_lock<std::shared_mutex> pk(c->pool_mu);
-    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
-    if (c->pool_used + T > kMaxPoolTets)
-        return fail(OCTMPS_E_RANGE, "resident meshes exceed 2^25 - 1 tets (packed neighbour ids)");
-    hipError_t e;
-    if (c->pool_used + T > c->pool_cap) {
-        int64_t ncap = std::max<int64_t>(c->pool_used + T, c->pool_cap * 2);
-        TetRec *np = nullptr;
-        if ((e = hipMalloc(&np, (size_t)ncap * sizeof(TetRec))) != hipSuccess)
-            return fail(OCTMPS_E_NOMEM, "mesh pool allocation failed");
-        if (c->pool_used > 0 &&
-            (e = hipMemcpy(np, c->pool_tets, (size_t)c->pool_used * sizeof(TetRec), hipMemcpyDeviceToDevice)) !=
-                hipSuccess) {
-            (void)hipFree(np);
-            return hip_fail(e, "mesh pool growth");
-        }
-        if (c->pool_tets) (void)hipFree(c->pool_tets);
-        c->pool_tets = np;
-        c->pool_cap = ncap;
-        if (c->graph.exec) {
-            (void)hipGraphExecDestroy(c->graph.exec);
-            c->graph.exec = nullptr;
-        }
-    }
-    const int64_t off = c->pool_used;
-    std::vector<TetRec> recs((size_t)T);
-    for (int64_t t = 0; t < T; ++t) {
-        TetRec &r = recs[t];
-        std::memset(&r, 0, sizeof r);
-        for (int j = 0; j < 4; ++j) {
-            const double *p = &m.planes[16 * t + 4 * j];
-            r.plane[j] = make_float4((float)p[0], (float)p[1], (float)p[2], (float)p[3]);
-        }
-        r.region = m.region[t];
-    }
-    for (int64_t t = 0; t < T; ++t) {
-        TetRec &r = recs[t];
-        for (int j = 0; j < 4; ++j) {
-            const int32_t l = m.links[4 * t + j];
-            if (l < 0) {
-                r.link[j] = 0xffffffffu;
-                continue;
-            }
-            // The face of l that links back to t through the exactly negated plane: a
-            // photon that crosses into l through it moves away from it (its cosine there is
-            // the negation of a negative one), so its next exit is one of the other three.
-            uint32_t back = 4;
-            for (int k = 0; k < 4 && back == 4; ++k) {
-                const float4 a = r.plane[j], b = recs[l].plane[k];
-                if (m.links[4 * (int64_t)l + k] == t && a.x == -b.x && a.y == -b.y && a.z == -b.z && a.w == -b.w)
-                    back = (uint32_t)k;
-            }
-            const uint32_t reg =
-                (back < 4 && m.region[l] < (int32_t)kRegionEscape) ? (uint32_t)m.region[l] : kRegionEscape;
-            r.link[j] = (reg << kRegionShift) | ((back & 3u) << kBackShift) | (uint32_t)(l + off);
-        }
-    }
-    if ((e = hipMemcpy(c->pool_tets + off, recs.data(), recs.size() * sizeof(TetRec), hipMemcpyHostToDevice)) !=
-        hipSuccess)
-        return hip_fail(e, "mesh upload");
-    MeshEntry me;
-    me.offset = off;
-    me.n_tets = T;
-    me.max_region = m.max_region;
-    me.region = m.region;
-    me.live = true;
-    c->meshes.push_back(me);
-    c->pool_used += T;
-    *mesh_id = (int32_t)c->meshes.size() - 1;
-    return OCTMPS_OK;
-}
-
-extern "C" octmps_status octmps_ctx_release_mesh(octmps_ctx *c, int32_t id) {
-    if (!c) return fail(OCTMPS_E_INVALID, "null argument");
-    std::unique_lock<std::shared_mutex> pk(c->pool_mu);
-    if (id < 0 || id >= (int)c->meshes.size() || !c->meshes[id].live) return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
-    c->meshes[id].live = false;
-    c->meshes[id].region.clear();
-    c->meshes[id].region.shrink_to_fit();
-    // The pool range is reclaimed when it is the last one (a stack of meshes).
-    while (!c->meshes.empty() && !c->meshes.back().live && c->meshes.back().offset + c->meshes.back().n_tets == c->pool_used) {
-        c->pool_used = c->meshes.back().offset;
-        c->meshes.back().n_tets = 0;
-        break;
-    }
-    return OCTMPS_OK;
-}
-
-extern "C" octmps_status octmps_ctx_memory(octmps_ctx *c, uint64_t *used, uint64_t *total) {
-    if (!c) return fail(OCTMPS_E_INVALID, "null argument");
-    uint64_t u = (uint64_t)c->pool_cap * sizeof(TetRec) + c->work.bytes();
-    if (used) *used = u;
-    if (total) *total = c->hbm_total;
-    return OCTMPS_OK;
 }
 
 extern "C" octmps_status octmps_run_ascan(octmps_ctx *c, int32_t mesh_id, const octmps_params *p,

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <atomic>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <shared_mutex>
 #include <string>
@@ -35,12 +38,22 @@ struct WorkBuffers {
     }
 };
 
+// A mesh known to the context. Pinned meshes (octmps_ctx_upload_mesh) stay resident in the
+// pool until released; registered meshes (octmps_ctx_register_mesh) keep their records on the
+// host and are made resident on demand (a run, a server's admission) and evicted when idle.
 struct MeshEntry {
-    int64_t offset = 0;   // first global tet id in the pool
+    int64_t offset = 0;   // first global tet id in the pool (while resident)
     int64_t n_tets = 0;
     int32_t max_region = 0;
-    bool live = false;
+    bool live = false;      // uploaded or registered, not released
+    bool resident = false;  // records in the pool
+    bool pinned = false;
+    std::atomic<int> refs{0};    // submitted requests not finished + runs in progress: no release
+    std::atomic<int> active{0};  // admitted requests + runs in progress: no eviction
+    double last_use = 0;
     std::vector<int32_t> region;  // host copy of tet regions (root region, Rspecular)
+    std::vector<TetRec> host;     // registered meshes: records with mesh-relative neighbour ids
+    uint64_t device_bytes() const { return (uint64_t)n_tets * sizeof(TetRec); }
 };
 
 struct GraphCache {
@@ -67,8 +80,9 @@ struct octmps_ctx {
     // Resident mesh pool (global tet ids).
     std::shared_mutex pool_mu;  // shared: launches and submits; exclusive: mesh upload/release
     octmps::TetRec *pool_tets = nullptr;
-    int64_t pool_cap = 0, pool_used = 0;
-    std::vector<octmps::MeshEntry> meshes;
+    int64_t pool_cap = 0;                  // tets the pool allocation holds
+    std::map<int64_t, int64_t> pool_free;  // free ranges below pool_cap: first tet -> count
+    std::vector<std::unique_ptr<octmps::MeshEntry>> meshes;
     // Plain/batched runs.
     std::mutex run_mu;
     octmps::WorkBuffers work;
@@ -89,6 +103,15 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
                         const octmps_ascan *scans, uint64_t seed, const octmps_batch_opts *opts,
                         uint64_t *tallies_out, octmps_run_stats *stats, PhotonLogRec *plog_out,
                         uint64_t plog_n, void *arena, size_t arena_bytes);
+// Mesh pool (mesh_pool.hip). The caller holds ctx->pool_mu exclusively.
+octmps_status mesh_make_resident(octmps_ctx *ctx, MeshEntry &me);
+void mesh_evict(octmps_ctx *ctx, MeshEntry &me);
+// Live mesh of an id, or nullptr (ctx->pool_mu held, shared or exclusive).
+MeshEntry *mesh_at(octmps_ctx *ctx, int32_t id);
+// For a plain or batched run: makes the mesh resident and counts the run as a user (refs and
+// active); mesh_release_run undoes the counts. Takes ctx->pool_mu exclusively itself.
+octmps_status mesh_acquire_run(octmps_ctx *ctx, int32_t id);
+void mesh_release_run(octmps_ctx *ctx, int32_t id);
 // Looks up the device address of the allocation-free path's scratch array (noalloc.hip);
 // called by octmps_ctx_create with the context's device current.
 octmps_status scratch_init(octmps_ctx *ctx);

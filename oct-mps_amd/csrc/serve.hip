@@ -1,9 +1,15 @@
 // Serving: continuous batching of A-scan / B-scan requests on one GPU.
 //
 // The reference runs one A-scan at a time from a host loop (src/main.cu:948-1051). Here a
-// scheduler thread keeps meshes resident and runs the transport kernel in slices: before
-// each slice it admits queued requests whose tallies fit the HBM budget and free scan
-// slots, and lists the photon ids of all admitted requests in the order of its hand-out
+// scheduler thread runs the transport kernel in slices: before each slice it admits queued
+// requests, first come first served, while their HBM footprint fits the server's budget -
+// the request's tallies, scan slots and parameter set, plus its mesh unless the mesh already
+// counts for an admitted request (photon streams are derived per photon, so no RNG state is
+// stored per request; the per-lane state of live photons is a fixed cost of the server). A
+// mesh that is not resident is uploaded on admission; meshes no admitted request uses are
+// kept resident while the budget allows and evicted, least recently used first, when an
+// admission needs the room (registered meshes only: pinned ones stay). It then lists the
+// photon ids of all admitted requests in the order of its hand-out
 // policy (least remaining photons first with aging, or equal shares; serve_order.hpp);
 // during the slice, lanes whose photon dies pick up the next listed id; at the end
 // of a slice live photons are saved per lane and resumed by the next slice
@@ -45,6 +51,10 @@ struct Req {
     uint64_t id = 0;
     int index = 0;
     int32_t mesh_id = 0;
+    MeshEntry *me = nullptr;  // the context's mesh entry (entries live as long as the context)
+    uint64_t charge = 0;      // bytes of the budget this request holds while admitted
+    bool counted = false;     // counted in me->refs (from submit to done)
+    bool mesh_active = false; // counted in me->active (from staging to done)
     octmps_params p;
     std::vector<octmps_ascan> scans;
     uint64_t seed = 0;
@@ -62,6 +72,20 @@ struct Req {
 struct Range {
     uint64_t off, len;
 };
+
+// A mesh counted in a server's budget: resident on its behalf, used by `users` admitted
+// requests (an idle one may be evicted when an admission needs the room).
+struct MeshUse {
+    uint64_t bytes = 0;
+    int users = 0;
+    double last_use = 0;
+};
+
+// Budget bytes of one request's device state besides its tallies: scan slots (the slot
+// record and its completion counter) and a parameter set with its region table.
+uint64_t request_state_bytes(size_t n_scans) {
+    return (uint64_t)n_scans * (sizeof(DevScan) + 8) + sizeof(DevSet) + kMaxRegions * sizeof(DevRegion);
+}
 
 }  // namespace
 }  // namespace octmps
@@ -96,7 +120,13 @@ struct octmps_server {
     uint64_t tally_cap_words = 0;
     bool have_saved = false;
     uint64_t last_claims = 0;  // photon ids claimed by the last slice
+    // HBM footprint admission
+    uint64_t budget = 0, used = 0;  // request footprints admitted (bytes) against the budget
+    uint64_t fixed_bytes = 0;       // the server's own device state, outside the budget
+    std::map<int32_t, MeshUse> mesh_use;
+    std::vector<int32_t> to_evict;  // meshes dropped from the budget, evicted before the next slice
     // stats
+    std::atomic<uint64_t> st_uploads{0}, st_evictions{0};
     uint64_t st_photons = 0, st_steps = 0, st_slices = 0;
     double st_kernel_ms = 0;
     int fatal = OCTMPS_OK;
@@ -105,15 +135,43 @@ struct octmps_server {
 namespace octmps {
 namespace {
 
+void free_tally(octmps_server *s, uint64_t off, uint64_t words);
+
+// Grows the device tally pool to hold at least `words` more (a new allocation; the live
+// tallies are copied over). Called between slices, from the scheduler thread.
+bool grow_tallies(octmps_server *s, uint64_t words) {
+    const uint64_t old = s->tally_cap_words;
+    const uint64_t ncap = std::max<uint64_t>(old * 2, old + words);
+    void *np = nullptr;
+    if (hipMalloc(&np, ncap * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (hipMemcpyAsync(np, s->tallies.p, old * 8, hipMemcpyDeviceToDevice, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess) {
+        (void)hipFree(np);
+        return false;
+    }
+    (void)hipFree(s->tallies.p);
+    s->tallies.p = np;
+    s->tallies.cap = ncap * 8;
+    s->tally_cap_words = ncap;
+    free_tally(s, old, ncap - old);
+    return true;
+}
+
 bool alloc_tally(octmps_server *s, uint64_t words, uint64_t &off) {
-    for (size_t i = 0; i < s->free_tally.size(); ++i) {
-        Range &r = s->free_tally[i];
-        if (r.len >= words) {
-            off = r.off;
-            r.off += words;
-            r.len -= words;
-            if (r.len == 0) s->free_tally.erase(s->free_tally.begin() + i);
-            return true;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && !grow_tallies(s, words)) return false;
+        for (size_t i = 0; i < s->free_tally.size(); ++i) {
+            Range &r = s->free_tally[i];
+            if (r.len >= words) {
+                off = r.off;
+                r.off += words;
+                r.len -= words;
+                if (r.len == 0) s->free_tally.erase(s->free_tally.begin() + i);
+                return true;
+            }
         }
     }
     return false;
@@ -139,7 +197,14 @@ void release(octmps_server *s, Req *r) {
         s->free_sets.push_back(r->set);
         std::sort(s->free_sets.begin(), s->free_sets.end(), std::greater<int>());
     }
-    if (r->admitted) free_tally(s, r->tally_base, r->words * r->scans.size());
+    if (r->admitted) {
+        free_tally(s, r->tally_base, r->words * r->scans.size());
+        MeshUse &u = s->mesh_use[r->mesh_id];
+        u.users--;
+        u.last_use = now_ms();
+        s->used -= r->charge;
+        r->charge = 0;
+    }
     r->set = -1;
     r->admitted = false;
     for (ScanState &x : r->ss) x.slot = -1;
@@ -150,7 +215,7 @@ void release(octmps_server *s, Req *r) {
 // cancel may already have lowered `limit`.
 octmps_status stage(octmps_server *s, Req *r) {
     octmps_ctx *ctx = s->ctx;
-    const MeshEntry &me = ctx->meshes[r->mesh_id];
+    const MeshEntry &me = *r->me;
     DevSet S;
     std::vector<DevRegion> R(kMaxRegions);
     octmps_status st = make_set(&r->p, me.max_region, r->set * kMaxRegions, S, R.data());
@@ -174,27 +239,67 @@ octmps_status stage(octmps_server *s, Req *r) {
     return OCTMPS_OK;
 }
 
-// Admission under the lock: FIFO while slots, a parameter set and tally space are free.
+// Ends a request that holds no server resources (cancelled or failed before admission, or
+// already released): drops its count on the mesh and wakes waiters. Under the lock.
+void end_unadmitted(octmps_server *s, Req *r, int status) {
+    r->status = status;
+    r->done = true;
+    r->t_done = now_ms() - s->t0;
+    if (r->counted) r->me->refs--;
+    r->counted = false;
+    s->cv_done.notify_all();
+}
+
+// Admission under the lock, first come first served: the head request is admitted when scan
+// slots and a parameter set are free and its footprint fits the budget, after dropping idle
+// meshes (no admitted user) from the budget, least recently used first, when it needs the
+// room; they are evicted from the device before the next slice. A request whose footprint
+// cannot fit even alone was refused at submit.
 void admit(octmps_server *s, std::vector<Req *> &staged) {
     while (!s->pending.empty()) {
         Req *r = s->pending.front();
         if (r->cancelled) {
             s->pending.pop_front();
-            r->status = OCTMPS_E_CANCELLED;
-            r->done = true;
-            r->t_done = now_ms() - s->t0;
-            s->cv_done.notify_all();
+            end_unadmitted(s, r, OCTMPS_E_CANCELLED);
             continue;
         }
-        size_t need = r->scans.size();
-        if (s->free_slots.size() < need || s->free_sets.empty()) break;
+        const size_t need_slots = r->scans.size();
+        if (s->free_slots.size() < need_slots || s->free_sets.empty()) break;
+        const uint64_t charge = r->words * need_slots * 8 + request_state_bytes(need_slots);
+        const bool counted_mesh = s->mesh_use.count(r->mesh_id) != 0;
+        const uint64_t mesh_bytes = counted_mesh ? 0 : r->me->device_bytes();
+        while (s->used + charge + mesh_bytes > s->budget) {
+            auto lru = s->mesh_use.end();
+            for (auto it = s->mesh_use.begin(); it != s->mesh_use.end(); ++it)
+                if (it->second.users == 0 && it->first != r->mesh_id &&
+                    (lru == s->mesh_use.end() || it->second.last_use < lru->second.last_use))
+                    lru = it;
+            if (lru == s->mesh_use.end()) break;
+            s->used -= lru->second.bytes;
+            s->to_evict.push_back(lru->first);
+            s->mesh_use.erase(lru);
+        }
+        if (s->used + charge + mesh_bytes > s->budget) break;  // waits for admitted requests to end
         uint64_t off;
-        if (!alloc_tally(s, r->words * need, off)) break;
+        if (!alloc_tally(s, r->words * need_slots, off)) {
+            if (s->active.empty()) {  // the device cannot hold its tallies at all
+                s->pending.pop_front();
+                end_unadmitted(s, r, OCTMPS_E_NOMEM);
+                continue;
+            }
+            break;
+        }
+        MeshUse &u = s->mesh_use[r->mesh_id];
+        if (!counted_mesh) u.bytes = mesh_bytes;
+        u.users++;
+        u.last_use = now_ms();
+        s->used += charge + mesh_bytes;
+        r->charge = charge;
         r->tally_base = off;
         r->admitted = true;
         r->set = s->free_sets.back();
         s->free_sets.pop_back();
-        for (size_t i = 0; i < need; ++i) {
+        for (size_t i = 0; i < need_slots; ++i) {
             r->ss[i].slot = s->free_slots.back();
             s->free_slots.pop_back();
         }
@@ -210,14 +315,49 @@ void finish(octmps_server *s, Req *r, int status) {
     r->done = true;
     r->t_done = now_ms() - s->t0;
     release(s, r);
+    if (r->mesh_active) r->me->active--;
+    r->mesh_active = false;
+    if (r->counted) r->me->refs--;
+    r->counted = false;
     s->active.erase(std::find(s->active.begin(), s->active.end(), r));
     s->cv_done.notify_all();
+}
+
+// Before a slice, with the pool held exclusively: evicts the meshes admission dropped from
+// the budget (unless another user holds them or they are pinned) and makes the meshes of
+// newly admitted requests resident. Returns the requests whose mesh could not be placed.
+std::vector<std::pair<Req *, octmps_status>> place_meshes(octmps_server *s, const std::vector<Req *> &staged,
+                                                          std::vector<int32_t> &evict) {
+    octmps_ctx *ctx = s->ctx;
+    std::vector<std::pair<Req *, octmps_status>> failed;
+    std::unique_lock<std::shared_mutex> pk(ctx->pool_mu);
+    for (int32_t id : evict) {
+        MeshEntry *me = mesh_at(ctx, id);
+        if (me && me->resident && !me->pinned && me->active.load() == 0) {
+            mesh_evict(ctx, *me);
+            s->st_evictions++;
+        }
+    }
+    for (Req *r : staged) {
+        const bool was = r->me->resident;
+        const octmps_status st = mesh_make_resident(ctx, *r->me);
+        if (st != OCTMPS_OK) {
+            failed.push_back({r, st});
+            continue;
+        }
+        if (!was) s->st_uploads++;
+        r->me->active++;
+        r->me->last_use = now_ms();
+        r->mesh_active = true;
+    }
+    return failed;
 }
 
 void scheduler(octmps_server *s) {
     octmps_ctx *ctx = s->ctx;
     (void)hipSetDevice(ctx->device);
     std::vector<uint64_t> done_h((size_t)s->n_slots);
+    std::vector<int32_t> evict;
     for (;;) {
         std::vector<Req *> staged;
         {
@@ -225,6 +365,18 @@ void scheduler(octmps_server *s) {
             while (!s->stop && s->pending.empty() && s->active.empty()) s->cv_work.wait(lk);
             if (s->stop && s->pending.empty() && s->active.empty()) break;
             admit(s, staged);
+            evict.swap(s->to_evict);
+        }
+        if (!staged.empty() || !evict.empty()) {
+            auto failed = place_meshes(s, staged, evict);
+            evict.clear();
+            if (!failed.empty()) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                for (auto &f : failed) {
+                    finish(s, f.first, f.second);
+                    staged.erase(std::find(staged.begin(), staged.end(), f.first));
+                }
+            }
         }
         std::shared_lock<std::shared_mutex> pk(ctx->pool_mu);
         for (Req *r : staged) {
@@ -252,7 +404,7 @@ void scheduler(octmps_server *s) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (Req *r : s->active) {
-                if (ctx->meshes[r->mesh_id].max_region >= kLdsSetRegions || n_lds == kLdsSets) sets_in_lds = false;
+                if (r->me->max_region >= kLdsSetRegions || n_lds == kLdsSets) sets_in_lds = false;
                 if (sets_in_lds) {
                     s->set_slot_h[r->set] = (uint8_t)n_lds;
                     s->lds_ids_h[n_lds++] = r->set;
@@ -303,7 +455,7 @@ void scheduler(octmps_server *s) {
         KArgs A;
         std::memset(&A, 0, sizeof A);
         A.tets = ctx->pool_tets;
-        A.tets_bytes = (uint32_t)(ctx->pool_used * (int64_t)sizeof(TetRec));
+        A.tets_bytes = (uint32_t)(ctx->pool_cap * (int64_t)sizeof(TetRec));
         A.regions = (const DevRegion *)s->regions.p;
         A.sets = (const DevSet *)s->sets.p;
         A.slots = (const DevScan *)s->slots.p;
@@ -356,9 +508,7 @@ void scheduler(octmps_server *s) {
             while (!s->pending.empty()) {
                 Req *r = s->pending.front();
                 s->pending.pop_front();
-                r->status = OCTMPS_E_DEVICE;
-                r->done = true;
-                s->cv_done.notify_all();
+                end_unadmitted(s, r, OCTMPS_E_DEVICE);
             }
             break;
         }
@@ -426,7 +576,7 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
     if (opts) s->o = *opts;
     s->ctx = ctx;
     s->t0 = now_ms();
-    s->n_slots = s->o.max_inflight_scans > 0 ? s->o.max_inflight_scans : 4096;
+    s->n_slots = s->o.max_inflight_scans > 0 ? s->o.max_inflight_scans : 32768;
     s->block = s->o.block_size > 0 ? s->o.block_size : 256;
     s->q = s->o.refill_quantum > 0 ? s->o.refill_quantum : 8;
     if (s->block % 64 != 0 || s->block > 256) {
@@ -448,8 +598,16 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
     grid_for(ctx, s->block, 0, true, s->grid);
     s->lanes = (uint64_t)s->grid * s->block;
     s->last_claims = s->lanes;
-    uint64_t budget = s->o.hbm_budget_bytes ? s->o.hbm_budget_bytes : (1ull << 30);
-    s->tally_cap_words = budget / 8;
+    // Default budget: three quarters of the device's memory. The budget covers the requests'
+    // footprints (meshes, tallies, per-request state); the server's fixed state (per-lane
+    // live photons and continuation photons, slot and set tables) comes on top of it.
+    s->budget = s->o.hbm_budget_bytes ? s->o.hbm_budget_bytes : ctx->hbm_total / 4 * 3;
+    s->fixed_bytes = s->lanes * (sizeof(SavedLane) + sizeof(ContRec)) +
+                     (uint64_t)s->n_slots * (sizeof(DevScan) + 8) +
+                     (uint64_t)kMaxSets * (sizeof(DevSet) + kMaxRegions * sizeof(DevRegion));
+    s->used = 0;
+    // The tally pool starts small and grows with the admitted requests.
+    s->tally_cap_words = (8ull << 20) / 8;
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&s->e0)) != hipSuccess || (e = hipEventCreate(&s->e1)) != hipSuccess ||
@@ -480,41 +638,54 @@ extern "C" octmps_status octmps_serve_submit(octmps_server *s, const octmps_requ
     if (!s || !rq || !rq->params || !rq->scans || rq->n_scans <= 0 || !rq->tallies_out)
         return fail(OCTMPS_E_INVALID, "bad request");
     octmps_ctx *ctx = s->ctx;
+    // The request counts as a user of its mesh from here until it is done, so the mesh cannot
+    // be released under it (octmps_ctx_release_mesh answers OCTMPS_E_BUSY meanwhile).
+    MeshEntry *me = nullptr;
+    const uint64_t words = tally_words(rq->params->num_depth_bins);
     {
         std::shared_lock<std::shared_mutex> pk(ctx->pool_mu);
-        if (rq->mesh_id < 0 || rq->mesh_id >= (int)ctx->meshes.size() || !ctx->meshes[rq->mesh_id].live)
-            return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
+        me = mesh_at(ctx, rq->mesh_id);
+        if (!me) return fail(OCTMPS_E_NOT_FOUND, "unknown mesh id");
         DevSet S;
         std::vector<DevRegion> R(kMaxRegions);
-        octmps_status st = make_set(rq->params, ctx->meshes[rq->mesh_id].max_region, 0, S, R.data());
+        octmps_status st = make_set(rq->params, me->max_region, 0, S, R.data());
         if (st != OCTMPS_OK) return st;
         for (int i = 0; i < rq->n_scans; ++i)
-            if (rq->scans[i].root_tet < 0 || rq->scans[i].root_tet >= ctx->meshes[rq->mesh_id].n_tets)
+            if (rq->scans[i].root_tet < 0 || rq->scans[i].root_tet >= me->n_tets)
                 return fail(OCTMPS_E_PROBE, "root tet out of range");
         if (!seed_valid(rq->seed, ctx->table->mult[0])) return fail(OCTMPS_E_SEED, "invalid seed");
+        const uint64_t footprint = words * (uint64_t)rq->n_scans * 8 + request_state_bytes((size_t)rq->n_scans) +
+                                   me->device_bytes();
+        if ((uint64_t)rq->n_scans > (uint64_t)s->n_slots)
+            return fail(OCTMPS_E_NOMEM, "request has more A-scans than the server has scan slots");
+        if (footprint > s->budget)
+            return fail(OCTMPS_E_NOMEM, "request's footprint (mesh " + std::to_string(me->device_bytes()) +
+                                            " B + tallies and state " + std::to_string(footprint - me->device_bytes()) +
+                                            " B) exceeds the server's HBM budget");
+        me->refs++;
     }
     Req *r = new (std::nothrow) Req();
-    if (!r) return fail(OCTMPS_E_NOMEM, "host allocation failed");
+    if (!r) {
+        me->refs--;
+        return fail(OCTMPS_E_NOMEM, "host allocation failed");
+    }
     r->mesh_id = rq->mesh_id;
+    r->me = me;
+    r->counted = true;
     r->p = *rq->params;
     r->scans.assign(rq->scans, rq->scans + rq->n_scans);
     r->seed = rq->seed;
     r->out = rq->tallies_out;
-    r->words = tally_words(rq->params->num_depth_bins);
+    r->words = words;
     r->ss.resize((size_t)rq->n_scans);
     for (int i = 0; i < rq->n_scans; ++i) r->ss[i].n = r->ss[i].limit = rq->scans[i].n_photons;
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop) {
+    if (s->stop || s->fatal != OCTMPS_OK) {
+        const int fatal = s->fatal;
+        me->refs--;
         delete r;
+        if (fatal != OCTMPS_OK) return fail((octmps_status)fatal, "server stopped after a device error");
         return fail(OCTMPS_E_BUSY, "server is closing");
-    }
-    if (s->fatal != OCTMPS_OK) {
-        delete r;
-        return fail((octmps_status)s->fatal, "server stopped after a device error");
-    }
-    if ((uint64_t)rq->n_scans > (uint64_t)s->n_slots || r->words * rq->n_scans > s->tally_cap_words) {
-        delete r;
-        return fail(OCTMPS_E_NOMEM, "request exceeds the server's slots or HBM budget");
     }
     r->id = s->next_id++;
     r->index = s->submits++;
@@ -594,6 +765,17 @@ extern "C" octmps_status octmps_serve_stats(octmps_server *s, uint64_t *photons,
     if (steps) *steps = s->st_steps;
     if (kernel_ms) *kernel_ms = s->st_kernel_ms;
     if (slices) *slices = s->st_slices;
+    return OCTMPS_OK;
+}
+
+extern "C" octmps_status octmps_serve_residency(octmps_server *s, uint64_t *uploads, uint64_t *evictions,
+                                                uint64_t *used_bytes, uint64_t *budget_bytes) {
+    if (!s) return fail(OCTMPS_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (uploads) *uploads = s->st_uploads.load();
+    if (evictions) *evictions = s->st_evictions.load();
+    if (used_bytes) *used_bytes = s->used;
+    if (budget_bytes) *budget_bytes = s->budget;
     return OCTMPS_OK;
 }
 

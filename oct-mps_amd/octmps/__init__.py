@@ -158,7 +158,9 @@ SIGNATURES = [
     ("octmps_ctx_create", C.c_int, [C.c_int32, C.POINTER(_P)]),
     ("octmps_ctx_destroy", None, [_P]),
     ("octmps_ctx_upload_mesh", C.c_int, [_P, _P, C.POINTER(C.c_int32)]),
+    ("octmps_ctx_register_mesh", C.c_int, [_P, _P, C.POINTER(C.c_int32)]),
     ("octmps_ctx_release_mesh", C.c_int, [_P, C.c_int32]),
+    ("octmps_ctx_mesh_info", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _u64p]),
     ("octmps_ctx_memory", C.c_int, [_P, _u64p, _u64p]),
     ("octmps_run_ascan", C.c_int, [_P, C.c_int32, C.POINTER(Params), C.POINTER(AScan), C.c_uint64,
                                    C.POINTER(BatchOpts), _u64p, C.POINTER(RunStats)]),
@@ -176,6 +178,7 @@ SIGNATURES = [
     ("octmps_serve_wait", C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(RequestInfo)]),
     ("octmps_serve_cancel", C.c_int, [_P, C.c_uint64]),
     ("octmps_serve_release", C.c_int, [_P, C.c_uint64]),
+    ("octmps_serve_residency", C.c_int, [_P, _u64p, _u64p, _u64p, _u64p]),
     ("octmps_serve_stats", C.c_int, [_P, _u64p, _u64p, C.POINTER(C.c_double), _u64p]),
     ("octmps_serve_close", C.c_int, [_P]),
     ("octmps_probe_rng", C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int32, C.c_int32,
@@ -428,8 +431,20 @@ class Context:
         check(lib.octmps_ctx_upload_mesh(self.h, mesh.h, C.byref(mid)), "upload_mesh")
         return mid.value
 
+    def register(self, mesh):
+        """Registers a mesh that is made resident on demand (octmps_ctx_register_mesh)."""
+        mid = C.c_int32(-1)
+        check(lib.octmps_ctx_register_mesh(self.h, mesh.h, C.byref(mid)), "register_mesh")
+        return mid.value
+
     def release(self, mesh_id):
         check(lib.octmps_ctx_release_mesh(self.h, mesh_id), "release_mesh")
+
+    def mesh_info(self, mesh_id):
+        """(resident, device bytes) of a mesh."""
+        res, nb = C.c_int32(0), C.c_uint64(0)
+        check(lib.octmps_ctx_mesh_info(self.h, mesh_id, C.byref(res), C.byref(nb)), "mesh_info")
+        return bool(res.value), nb.value
 
     def run_ascan(self, mesh_id, params, scan, seed, opts=None):
         bins = params.num_depth_bins
@@ -547,6 +562,12 @@ class Server:
         """Forget a finished request (octmps_serve_release) and drop its output buffer."""
         check(lib.octmps_serve_release(self.h, rid), "serve_release")
         self._keep.pop(rid, None)
+
+    def residency(self):
+        up, ev, used, budget = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib.octmps_serve_residency(self.h, C.byref(up), C.byref(ev), C.byref(used), C.byref(budget)),
+              "serve_residency")
+        return {"uploads": up.value, "evictions": ev.value, "used": used.value, "budget": budget.value}
 
     def stats(self):
         ph, st, sl = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -34,7 +34,12 @@
 // roulette with the continuation-photon restore (:1376-1391) on enumerated states, one per
 // side of each condition (tests/test_cpu_oracle_branches.py); the whole loop against a
 // glass-slab known answer. The reference ships no golden outputs and its kernel cannot run
-// here, so none of these is compared with reference output.
+// here, so none of these is compared with reference output. ComputeStepSize and Drop are
+// pinned through event traces (tests/test_cpu_oracle_transport_pins.py): the optical depth of
+// every complete free path follows Exp(1) truncated at the ray's optical distance to the mesh
+// boundary, across faces between regions of different mu_t, and the weight ratio between
+// consecutive scatterings is mus/(mua+mus). The ORACLE_PLANT_* macros build deliberately
+// wrong variants that those tests must reject; liboracle.so is built without them.
 #include <atomic>
 #include <cfloat>
 #include <cmath>
@@ -180,6 +185,9 @@ struct Photon {  // PhotonStructGPU, src/octmps_kernel.h:219-255
     uint32_t hit, dead, FstBackReflectionFlag, rootIdx, NumBackwardsSpecularReflections;
     double x, y, z, ux, uy, uz, w, s, sleft, MinCos, LocationFstBias, OpticalPath, MaxDepth;
     double LikelihoodRatio, LikelihoodRatioAfterFstBias;
+#ifdef ORACLE_PLANT_LEFTOVER_OLD_REGION
+    double plant_mut;
+#endif
 };
 
 struct Tally {
@@ -233,7 +241,12 @@ void compute_step(const Sim &s, Photon &p, Rng &g) {
         double r = rand_oc(g);
         p.s = -log(r) * s.reg[region_of(s, p.rootIdx)].rmuas;
     } else {
+#ifdef ORACLE_PLANT_LEFTOVER_OLD_REGION
+        // Planted mistake (pin tests only): the leftover converted with the old region's mu_t.
+        p.s = p.sleft / p.plant_mut;
+#else
         p.s = p.sleft * s.reg[region_of(s, p.rootIdx)].rmuas;
+#endif
         p.sleft = 0.0;
     }
 }
@@ -261,6 +274,9 @@ uint32_t hit_boundary(const Sim &s, Photon &p) {
     if (p.s > min_distance) {
         double mut = s.reg[region_of(s, t)].muas;
         p.sleft = (p.s - min_distance) * mut;
+#ifdef ORACLE_PLANT_LEFTOVER_OLD_REGION
+        p.plant_mut = mut;
+#endif
         p.s = min_distance;
         p.NextTetrahedron = imin;
         return 1;
@@ -280,7 +296,14 @@ void hop(Photon &p) {
 
 // Drop, src/octmps_kernel.cuh:584-588.
 void drop(const Sim &s, Photon &p) {
+#ifdef ORACLE_PLANT_DROP_MUA_OVER_MUS
+    // Planted mistake (pin tests only): absorbed fraction mua/mus instead of mua/(mua+mus).
+    const Region &r = s.reg[region_of(s, p.rootIdx)];
+    const double mua = r.mua_muas * r.muas, mus = r.muas - mua;
+    double dwa = p.w * (mua / mus);
+#else
     double dwa = p.w * s.reg[region_of(s, p.rootIdx)].mua_muas;
+#endif
     p.w -= dwa;
 }
 

@@ -3,6 +3,7 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the CPU-baseline
 leg of bench.py, always as the checker, never as the measured or shipped path.
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -20,42 +21,58 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             raise ImportError(f"oracle not built ({LIB}); run __graft_entry__.build()")
-        L = C.CDLL(LIB)
-        dp, ip, up = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)
-        u32p = C.POINTER(C.c_uint32)
-        L.oracle_run_ascan.restype = C.c_int
-        L.oracle_run_ascan.argtypes = [dp, ip, ip, C.c_int64, C.c_void_p, C.c_void_p, C.c_uint64, u32p, C.c_uint32,
-                                       C.c_uint64, C.c_int, dp, up, dp]
-        L.oracle_trace_photon.restype = C.c_int
-        L.oracle_trace_photon.argtypes = [dp, ip, ip, C.c_int64, C.c_void_p, C.c_void_p, C.c_uint64, u32p,
-                                          C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, dp]
-        L.oracle_div_rd.restype = C.c_double
-        L.oracle_div_rd.argtypes = [C.c_double, C.c_double]
-        L.oracle_rand_co.restype = None
-        L.oracle_rand_co.argtypes = [up, C.c_uint32, C.c_int, dp]
-        L.oracle_photon_streams.restype = None
-        L.oracle_photon_streams.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, u32p, C.c_uint32, up, u32p]
-        L.oracle_fresnel.restype = C.c_double
-        L.oracle_fresnel.argtypes = [C.c_double, C.c_double, C.c_double]
-        L.oracle_spin_theta.restype = C.c_double
-        L.oracle_spin_theta.argtypes = [C.c_double, C.c_double]
-        L.oracle_spin_theta_bias.restype = C.c_double
-        L.oracle_spin_theta_bias.argtypes = [C.c_double, C.c_double]
-        L.oracle_hit_boundary.restype = None
-        L.oracle_hit_boundary.argtypes = [dp, dp, dp, C.c_double, C.c_double, dp]
-        L.oracle_spin_bias_batch.restype = None
-        L.oracle_spin_bias_batch.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_uint64, u32p,
-                                             C.c_uint32, C.c_int, dp]
-        L.oracle_tally_detection.restype = None
-        L.oracle_tally_detection.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
-                                             C.c_double, dp, up, up]
-        L.oracle_detect_filter.restype = C.c_int
-        L.oracle_detect_filter.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
-        L.oracle_roulette_batch.restype = None
-        L.oracle_roulette_batch.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_uint64, u32p, C.c_uint32, C.c_int,
-                                            dp]
-        _lib = L
+        _lib = _configure(C.CDLL(LIB))
     return _lib
+
+
+@contextlib.contextmanager
+def variant(path):
+    """Runs the block with another build of oracle.cpp (a planted-mistake variant that a pin
+    test must reject), then restores the oracle."""
+    global _lib
+    saved = lib()
+    _lib = _configure(C.CDLL(path))
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def _configure(L):
+    dp, ip, up = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)
+    u32p = C.POINTER(C.c_uint32)
+    L.oracle_run_ascan.restype = C.c_int
+    L.oracle_run_ascan.argtypes = [dp, ip, ip, C.c_int64, C.c_void_p, C.c_void_p, C.c_uint64, u32p, C.c_uint32,
+                                   C.c_uint64, C.c_int, dp, up, dp]
+    L.oracle_trace_photon.restype = C.c_int
+    L.oracle_trace_photon.argtypes = [dp, ip, ip, C.c_int64, C.c_void_p, C.c_void_p, C.c_uint64, u32p,
+                                      C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, dp]
+    L.oracle_div_rd.restype = C.c_double
+    L.oracle_div_rd.argtypes = [C.c_double, C.c_double]
+    L.oracle_rand_co.restype = None
+    L.oracle_rand_co.argtypes = [up, C.c_uint32, C.c_int, dp]
+    L.oracle_photon_streams.restype = None
+    L.oracle_photon_streams.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, u32p, C.c_uint32, up, u32p]
+    L.oracle_fresnel.restype = C.c_double
+    L.oracle_fresnel.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.oracle_spin_theta.restype = C.c_double
+    L.oracle_spin_theta.argtypes = [C.c_double, C.c_double]
+    L.oracle_spin_theta_bias.restype = C.c_double
+    L.oracle_spin_theta_bias.argtypes = [C.c_double, C.c_double]
+    L.oracle_hit_boundary.restype = None
+    L.oracle_hit_boundary.argtypes = [dp, dp, dp, C.c_double, C.c_double, dp]
+    L.oracle_spin_bias_batch.restype = None
+    L.oracle_spin_bias_batch.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_uint64, u32p,
+                                         C.c_uint32, C.c_int, dp]
+    L.oracle_tally_detection.restype = None
+    L.oracle_tally_detection.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, dp, up, up]
+    L.oracle_detect_filter.restype = C.c_int
+    L.oracle_detect_filter.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
+    L.oracle_roulette_batch.restype = None
+    L.oracle_roulette_batch.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_uint64, u32p, C.c_uint32, C.c_int,
+                                        dp]
+    return L
 
 
 def _dp(a):

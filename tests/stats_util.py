@@ -4,10 +4,12 @@ import numpy as np
 
 def bin_stats(total, total_sq, n):
     """Mean and standard error per bin with the reference's estimator
-    (WriteReflectanceClassI_and_ClassII, src/octmps_io.cpp:309-318)."""
+    (WriteReflectanceClassI_and_ClassII, src/octmps_io.cpp:309-318), unclamped: a negative
+    variance gives NaN, as in the reference's output."""
     mean = total / n
     var = (total_sq / n - mean * mean) / (n - 1)
-    return mean, np.sqrt(np.maximum(var, 0.0))
+    with np.errstate(invalid="ignore"):
+        return mean, np.sqrt(var)
 
 
 def filtered(t, max_rel=0.01):
@@ -36,6 +38,10 @@ def compare(gpu, ref, n):
     5 sigma_comb, and total R (sum over bins of Class I + Class II means) within 0.5%."""
     mg, sg = bin_stats(gpu["sum"], gpu["sumsq"], n)
     mr, sr = bin_stats(ref["sum"], ref["sumsq"], n)
+    # Every bin's SE must be finite on both sides (the exact tally holds every contribution's
+    # square, so a negative variance would be a tally fault, not noise).
+    assert np.all(np.isfinite(sg)), f"GPU SE not finite in {int((~np.isfinite(sg)).sum())} bins"
+    assert np.all(np.isfinite(sr)), f"oracle SE not finite in {int((~np.isfinite(sr)).sum())} bins"
     sig = np.sqrt(sg ** 2 + sr ** 2)
     pop = sig > 0
     z = np.zeros_like(mg)

@@ -303,3 +303,31 @@ def test_launch_shape_never_changes_results(ctx, sq, q, block):
     got, st = ctx.run_bscan(mid, params, scans, seed, opts=o)
     assert st.block_size == block and st.refill_quantum == q
     assert np.array_equal(ref, got)
+
+
+@pytest.mark.parametrize("name", ["slab4_full", "bump_full"])
+def test_full_size_benchmark_meshes_parity_paired(ctx, table, name):
+    # The benchmark meshes themselves, not coarsened: the four-layer slab (64x64x32 cells,
+    # 786,432 tets, 101 MB of records, kernel with the entry-plane skip) and the bump
+    # (128x128x64 cells, 6,291,456 tets, 805 MB: larger than the Infinity Cache, so the launch
+    # takes the variant without the skip). One A-scan at the centre, 10,000 paired photons.
+    n = 10000
+    if name == "slab4_full":
+        mesh, params, scans, seed = configs.slab4(n_scans=1, photons=n)
+        assert mesh.num_tets == 786432
+    else:
+        mesh, params, scans, seed = configs.bump(n_scans=1, photons=n)
+        assert mesh.num_tets == 6291456
+    gpu, st, plog, ref = _run_pair(ctx, table, mesh, params, scans[0], seed, threads=16)
+    assert st.photons == n and st.killed == 0 and ref["killed"] == 0 and st.bad_contributions == 0
+    assert ref["num_filtered"] > 100
+    same_steps = np.mean(plog["steps"].astype(np.float64) == ref["log"][:, 0])
+    c = compare(gpu, ref, n)
+    print(name, c, "same steps", same_steps, "steps/photon", st.photon_steps / n)
+    # Pairing check: unpaired streams would give ~0 photons with the same step count. On these
+    # fine meshes a photon takes ~2e4 steps, so its FP32 and FP64 paths split at some near-edge
+    # crossing more often than on the coarse meshes above (0.35 of photons keep the same count
+    # on the slab).
+    assert same_steps > 0.2
+    assert c["frac_3sigma"] <= 0.01 and c["max_abs_z"] < 5
+    assert c["R_rel"] < 0.005 and c["Rfilt_rel"] < 0.005
