@@ -54,6 +54,44 @@ int ref_read_inputs(const char *opt, const char *mci, double *out, int cap) {
     return k;
 }
 
+// read_simulation_data for run `run` of a multi-run .opt: out[0..] = n_regions, then per
+// region (n, mua, mus, g, mutr), photons, probe start/step/end. Returns the number of values.
+// read_simulation_data stores run i's probe line through simulations[i] - the i-th pointer
+// after the one it fills with the run array (src/octmps_io.cpp:636-640) - so it is given an
+// array of pointers whose entries 1.. point to structs that receive those probe lines.
+int ref_read_run(const char *opt, int run, double *out, int cap) {
+    constexpr int kMaxRuns = 64;
+    static SimulationStruct probe_sinks[kMaxRuns];
+    SimulationStruct *ptrs[kMaxRuns];
+    ptrs[0] = nullptr;
+    for (int i = 1; i < kMaxRuns; ++i) ptrs[i] = &probe_sinks[i];
+    // The run count is the second data line; refuse files with more runs than sinks.
+    int n = read_simulation_data((char *)opt, ptrs);
+    if (n <= 0 || n > kMaxRuns) return -1;
+    if (run < 0 || run >= n) return -3;
+    const SimulationStruct &s = ptrs[0][run];
+    const SimulationStruct &pr = run == 0 ? ptrs[0][0] : probe_sinks[run];
+    int k = 0;
+    auto put = [&](double v) {
+        if (k < cap) out[k] = v;
+        ++k;
+    };
+    put(s.n_regions);
+    for (unsigned i = 0; i < s.n_regions; ++i) {
+        put(s.regions[i].n);
+        put(i ? s.regions[i].mua : 0);
+        put(i ? s.regions[i].mus : 0);
+        put(i ? s.regions[i].g : 0);
+        put(i ? s.regions[i].mutr : 0);
+    }
+    put(s.number_of_photons);
+    put(pr.probe.start_x);
+    put(pr.probe.distance_Ascans);
+    put(pr.probe.end_x);
+    FreeSimulationStruct(ptrs[0], n);
+    return k;
+}
+
 // MeshGraph + FindRootofGraph + SerializeGraph for a probe at (x, 0, 0).
 // planes[T*16] = d_faces[faces[f]].N*signs[f], d*signs[f]; links[T*4]; regions[T].
 // Returns the root index, or -1 (no root) / -2 (bad mesh).

@@ -129,3 +129,38 @@ def test_writer_byte_identical_to_reference(tmp_path):
     for f in ("BiasParam.mco", "ClassI_Scatt.out", "ClassII_Scatt.out", "ClassI_ScattFilt.out",
               "ClassII_ScattFilt.out", "FilteredPhotons.mco"):
         assert filecmp.cmp(mine / f, theirs / f, shallow=False), f
+
+
+TWO_RUNS = ("1.0  # version\n2  # runs\n"
+            "run0.mco A\n3000  # photons\n2  # regions\n1.0  # ambient n\n"
+            "1.40 1.0 100 0.9\n1.33 0.4 60 0.9\n-0.002 0.002 0.003  # start step end\n"
+            "run1.mco A\n1500\n3\n1.0\n1.38 0.5 80 0.85\n1.36 0.2 60 0.9\n1.33 0.1 50 0.9\n"
+            "-0.001 0.001 0.001\n")
+
+
+@needs_ref
+def test_multi_run_opt_parses_like_reference(tmp_path):
+    # Each run of a multi-run .opt parses to the reference's regions, photon count and probe
+    # line for that run (read_simulation_data, src/octmps_io.cpp:504-645).
+    opt = tmp_path / "two.opt"
+    opt.write_text(TWO_RUNS)
+    L = C.CDLL(OR.REF_LIB)
+    L.ref_read_run.restype = C.c_int
+    L.ref_read_run.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_double), C.c_int]
+    for run in (0, 1):
+        buf = np.zeros(256, np.float64)
+        n = L.ref_read_run(str(opt).encode(), run, buf.ctypes.data_as(C.POINTER(C.c_double)), 256)
+        assert n > 0
+        p, runs = O.read_opt(str(opt), run)
+        assert runs == 2
+        mine = [p.n_regions]
+        for i in range(p.n_regions):
+            r = p.regions[i]
+            mine += [r.n, r.mua if i else 0, r.mus if i else 0, r.g if i else 0,
+                     (1.0 / (r.mua + r.mus)) if i else 0]
+        mine += [p.number_of_photons, p.probe_start_x, p.probe_step_x, p.probe_end_x]
+        assert len(mine) == n
+        assert np.array_equal(np.array(mine), buf[:n]), run
+    with pytest.raises(O.OctmpsError) as e:
+        O.read_opt(str(opt), 2)
+    assert e.value.status == 5

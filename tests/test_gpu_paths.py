@@ -326,6 +326,41 @@ def test_cli_writes_reference_outputs(tmp_path, ctx):
     assert np.allclose(got, out[0, 0, 0], rtol=1e-5, atol=0)
 
 
+def test_cli_runs_every_run_of_a_multi_run_opt(tmp_path, ctx):
+    # A 2-run .opt: at each probe position of run 0's line, run 0 then run 1 are simulated with
+    # their own regions and photon counts and appended in that order (src/main.cu:948-1036).
+    from test_cpu_formats import TWO_RUNS
+    mesh = O.Mesh.generate(O.GEOM_LAYERS, (16, 16, 8), (0.2, 0.2, 0.1), p=(0.02,), n_layers=2)
+    _write_case(tmp_path, mesh)
+    (tmp_path / "case.opt").write_text(TWO_RUNS)
+    r = subprocess.run([O.CLI_PATH, "-S985456376", "case.opt", "case.mesh", "case.mci"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    p0, runs = O.read_opt(str(tmp_path / "case.opt"), 0)
+    assert runs == 2
+    O.read_mci(str(tmp_path / "case.mci"), p0)
+    p1 = type(p0).from_buffer_copy(p0)
+    O.read_opt(str(tmp_path / "case.opt"), 1, p1)
+    B = p0.num_depth_bins
+    rows = [x.split(",") for x in (tmp_path / "ClassI_Scatt.out").read_text().splitlines()]
+    xs = [-0.002, 0.0, 0.002]  # run 0's line; run 1's own line is not the B-scan's
+    assert len(rows) == len(xs) * 2 * B
+    m2 = O.Mesh.read(str(tmp_path / "case.mesh"))
+    mid = ctx.upload(m2)
+    for i, x in enumerate(xs):
+        for k, (pk, nk) in enumerate(((p0, 3000), (p1, 1500))):
+            block = rows[(2 * i + k) * B:(2 * i + k + 1) * B]
+            assert all(abs(float(row[0]) - x) < 1e-12 for row in block)
+            if i == 0:
+                root, _ = m2.find_root(x, 0.0, 0.0)
+                t, st = ctx.run_ascan(mid, pk, O.ascan(x, 0.0, 0.0, root, nk), 985456376)
+                assert st.photons == nk
+                want = O.finalize(t, B, nk, pk.max_relative_contribution)
+                got = np.array([float(row[2]) for row in block])
+                assert np.allclose(got, want[0, 0, 0], rtol=1e-5, atol=0), k
+    assert "1500" in (tmp_path / "BiasParam.mco").read_text()  # rewritten by the last run
+
+
 def test_integration_python_example_runs(tmp_path, ctx, monkeypatch):
     # The Python example of INTEGRATION.md, run as written on a small two-layer case: its
     # A-scan equals the library's own plain run of the same inputs.
