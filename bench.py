@@ -25,20 +25,24 @@ ALG_BYTES_PER_STEP = 84  # one tet record per photon-step: 4 planes x 16 B + 4 l
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E nominal
 LAYOUT_BYTES_PER_STEP = 80  # this build's packed tet record: 4 planes x 16 B + 4 packed neighbours x 4 B
 
-# Counter-derived figures of the batched slab4 launch (128 A-scans x 1e6 photons), each from
-# the rocprofv3 summary named beside it; per photon-step = counter / the launch's photon-steps.
+# Counter-derived figures of the batched slab4 launch (128 A-scans x 1e6 photons, quanta 24/20),
+# with the entry-plane skip on and off, each from the rocprofv3 summary named beside it; per
+# photon-step = counter / the launch's photon-steps.
 PROFILE = {
+    "skip_on": {"lane_loads_per_step": 4.16, "ta_busy": 0.77, "td_busy": 1.00, "valu_issue_upper": 1.10,
+                "valu_per_step": 5.75, "source": "profiles/transport_slab4_skip_counters.md"},
+    "skip_off": {"lane_loads_per_step": 5.01, "ta_busy": 0.85, "td_busy": 1.00, "valu_issue_upper": 1.04,
+                 "valu_per_step": 5.47, "source": "profiles/transport_slab4_memory.md, transport_slab4_sq.md"},
     # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors by which those counters read a copy
-    # kernel of known size), per photon-step.
-    "hbm_bytes_per_step": 26.09, "hbm_source": "profiles/transport_slab4_pmc.md",
-    # TD_TD_BUSY_sum / (CUs x launch cycles per CU); TA_BUSY_avr / launch cycles per CU.
-    "td_busy": 1.00, "ta_busy": 0.76, "mem_source": "profiles/transport_slab4_memory.md",
-    # SQ_INSTS_VALU x 4 cycles / (SIMDs x launch cycles): an upper bound on VALU issue.
-    "valu_issue_upper": 1.07, "valu_per_step": 5.58, "sq_source": "profiles/transport_slab4_sq.md",
-    # TCP_TOTAL_CACHE_ACCESSES_sum / photon-steps: 16-byte lane-loads of the tet record (5 per
-    # visit, 4 after entering through a face) plus the other vector accesses.
-    "lane_loads_per_step": 4.16,
+    # kernel of known size), per photon-step (skip off).
+    "hbm_bytes_per_step": 25.58, "hbm_source": "profiles/transport_slab4_pmc.md",
+    # The bump sample (64 of 1024 A-scans x 1e6 photons, quanta 40/16, skip off): HBM bytes
+    # per photon-step from its own FETCH_SIZE/WRITE_SIZE passes.
+    "bump_hbm_bytes_per_step": 47.35, "bump_hbm_source": "profiles/transport_bump_pmc.md",
 }
+# Records of a mesh up to this size stay in the Infinity Cache; the entry-plane skip defaults
+# to on for them (entry_skip 0 = auto).
+INFINITY_CACHE_BYTES = 256 << 20
 
 
 def log(msg):
@@ -65,6 +69,8 @@ def parse():
     ap.add_argument("--quanta", default="",
                     help="scatter,refill quanta to launch with instead of autotuning (e.g. 36,16: the same "
                          "launch a profiler run can then attribute to one kernel dispatch)")
+    ap.add_argument("--entry-skip", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="with --quanta: entry-plane skip of the launch (0 auto, 1 on, 2 off)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate of the C3 stream, req/s")
     ap.add_argument("--serve-window", type=float, default=0.1, help="seconds of arrivals in the C3 stream")
     ap.add_argument("--sweep-rates", default="10,25,50,100,200",
@@ -86,6 +92,8 @@ def parse():
     ap.add_argument("--stride", type=int, default=1,
                     help="keep every k-th A-scan of the workload (a bounded sample of a long B-scan)")
     ap.add_argument("--no-probes", action="store_true", help="skip the gather and HBM copy probes")
+    ap.add_argument("--bump-stride", type=int, default=16,
+                    help="slab4 runs also time the bump mesh on every k-th of its 1024 A-scans (0: skip)")
     return ap.parse_args()
 
 
@@ -235,7 +243,10 @@ def main():
     div = max(1, a.slice_div)
     if a.quanta:
         opts.scatter_quantum, opts.refill_quantum = (int(v) for v in a.quanta.split(","))
-        tuned = {"scatter_quantum": opts.scatter_quantum, "refill_quantum": opts.refill_quantum, "source": "--quanta"}
+        if a.entry_skip >= 0:
+            opts.entry_skip = a.entry_skip
+        tuned = {"scatter_quantum": opts.scatter_quantum, "refill_quantum": opts.refill_quantum,
+                 "entry_skip": opts.entry_skip, "source": "--quanta"}
     elif not a.no_autotune:
         # The autotuner (coordinate search over launch options on a calibration slice of the
         # same B-scan: the first 1/div of every A-scan's photons), before the timed region;
@@ -356,6 +367,34 @@ def main():
         hbm_copy = 2 * nb / (ms * 1e-3) / 1e9
         log(f"gather ceiling {gather_peak:.3e} lane-loads/s; HBM copy {hbm_copy:.0f} GB/s (read + write)")
 
+    # The bump mesh (6.29 M tets, 805 MB of records: larger than the Infinity Cache, the one
+    # workload where HBM can bind) on a sample of its B-scan, with the profiled launch options.
+    bump = None
+    if a.workload == "slab4" and a.bump_stride > 0:
+        ts = time.time()
+        bm, bpar, bscans, bseed = configs.bump()
+        bscans = bscans[::a.bump_stride]
+        bmid = ctx.upload(bm)
+        bo = O.BatchOpts()
+        bo.use_graph, bo.scatter_quantum, bo.refill_quantum, bo.entry_skip = 1, 40, 16, 2
+        _, bst = ctx.run_bscan(bmid, bpar, bscans, bseed, opts=bo)
+        ctx.release(bmid)
+        del bm
+        bsteps, bms = bst.photon_steps, bst.kernel_ms
+        b_alg = bsteps * ALG_BYTES_PER_STEP / (bms * 1e-3) / 1e9
+        b_hbm = bsteps * PROFILE["bump_hbm_bytes_per_step"] / (bms * 1e-3) / 1e9
+        bump = {"workload": f"bump: 6x6x3 mm, 128x128x64 cells x 6 tets (6,291,456 tets), {len(bscans)} of 1024 "
+                            f"A-scans x 1e6 photons, 1024 bins to 2 mm, quanta 40/16, entry skip off",
+                "photons": bst.photons, "kernel_ms": bms, "photons_per_s": bst.photons / (bms * 1e-3),
+                "photon_steps": bsteps, "steps_per_photon": bsteps / bst.photons,
+                "roofline": {"bound": "hbm", "achieved": b_alg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": b_alg / HBM_PEAK_GBS, "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step",
+                             "measured_hbm": {"bytes_per_step": PROFILE["bump_hbm_bytes_per_step"], "GB_s": b_hbm,
+                                              "frac": b_hbm / HBM_PEAK_GBS,
+                                              "source": PROFILE["bump_hbm_source"]}}}
+        log(f"bump sample: {bst.photons} photons in {bms:.0f} ms kernel ({bump['photons_per_s']:.3e} photons/s), "
+            f"HBM {b_hbm:.0f} GB/s measured-bytes basis ({time.time() - ts:.0f} s with the mesh build)")
+
     # Timed steps: full B-scans, as many of the requested ones as fit in the budget.
     walls, kms, steps, iters, bad = [], [], [], [], 0
     t = None
@@ -387,12 +426,14 @@ def main():
     if served_same is not None:
         served_same["equal_to_batched"] = bool(np.array_equal(served_out, t))
 
+    skip_on = opts.entry_skip == 1 or (opts.entry_skip == 0 and mesh.num_tets * 128 <= INFINITY_CACHE_BYTES)
+    prof = PROFILE["skip_on" if skip_on else "skip_off"]
     gather = None
     if gather_peak:
-        got = sum(steps) / kernel_s * PROFILE["lane_loads_per_step"]
+        got = sum(steps) / kernel_s * prof["lane_loads_per_step"]
         gather = {"bound": "vector-memory gathers", "achieved": got, "peak": gather_peak,
                   "unit": "16-byte lane-loads/s", "frac": got / gather_peak,
-                  "lane_loads_per_step": PROFILE["lane_loads_per_step"],
+                  "lane_loads_per_step": prof["lane_loads_per_step"], "entry_skip": skip_on,
                   "peak_source": "octmps_probe_gather: 5 x 16 B from pseudo-random 128-byte records of a "
                                  "2 MiB set, 6 waves per SIMD"}
     ctx.close()
@@ -434,11 +475,13 @@ def main():
                      "layout_bytes_per_step": LAYOUT_BYTES_PER_STEP,
                      "measured_hbm": {"bytes_per_step": PROFILE["hbm_bytes_per_step"], "GB_s": hbm_measured,
                                       "frac": hbm_measured / HBM_PEAK_GBS, "source": PROFILE["hbm_source"]}},
-        "bound_by": {"resource": "TD (vector-memory data return of the divergent tet-record gathers)",
-                     "frac": PROFILE["td_busy"], "source": PROFILE["mem_source"],
-                     "ta_busy": PROFILE["ta_busy"], "valu_issue_upper_bound": PROFILE["valu_issue_upper"],
-                     "valu_source": PROFILE["sq_source"], "lane_utilisation": lane_util},
+        "bound_by": {"resource": "vector-memory gather path (TD busy for the whole launch) together with VALU issue",
+                     "frac": prof["td_busy"], "gather_frac": gather["frac"] if gather else None,
+                     "ta_busy": prof["ta_busy"], "valu_issue_upper_bound": prof["valu_issue_upper"],
+                     "valu_per_step": prof["valu_per_step"], "source": prof["source"],
+                     "entry_skip": skip_on, "lane_utilisation": lane_util},
         "gather_roofline": gather,
+        "bump_sample": bump,
         "cpu_baseline": cpu,
         "serving": serving,
         "served_same_bscan": served_same,

@@ -310,7 +310,9 @@ octmps_status octmps_run_bscan_logged(octmps_ctx *ctx, int32_t mesh_id, const oc
                                       octmps_run_stats *stats, void *photon_log,
                                       uint64_t log_records);
 /* Autotune scatter quantum, block size, refill quantum, tally privatisation depth and the
- * entry-plane skip on a calibration slice of a B-scan (coordinate search, 25 launches);
+ * entry-plane skip on a calibration slice of a B-scan (coordinate search from the context
+ * defaults, 18 launches: a candidate must be more than 1 % faster to win, and the winner must
+ * lead the starting point again on a second measurement of both);
  * writes the winner into *opts (scatter_quantum, block_size, refill_quantum, tally_copies,
  * entry_skip), makes the first four the context's default (entry_skip stays per launch, since
  * its best value depends on the mesh), and returns its photons/s in *rate. */

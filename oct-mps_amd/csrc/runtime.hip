@@ -143,10 +143,10 @@ octmps_status make_scan(octmps_ctx *ctx, const MeshEntry &me, const octmps_param
 }
 
 int default_block(octmps_ctx *ctx) { return ctx->tuned_block > 0 ? ctx->tuned_block : 256; }
-// Untuned defaults: measured best on the four-layer slab B-scan (128 x 3e5 photons), where
-// fuller refills and scatter phases beat the lane utilisation they give up.
-int default_quantum(octmps_ctx *ctx) { return ctx->tuned_q > 0 ? ctx->tuned_q : 16; }
-int default_scatter(octmps_ctx *ctx) { return ctx->tuned_sq > 0 ? ctx->tuned_sq : 40; }
+// Untuned defaults: measured best on the four-layer slab B-scan (128 A-scans x 1e5 photons:
+// scatter/refill quanta 24/20 in 2334 ms, 32/16 in 2357 ms, 40/16 in 2446 ms).
+int default_quantum(octmps_ctx *ctx) { return ctx->tuned_q > 0 ? ctx->tuned_q : 20; }
+int default_scatter(octmps_ctx *ctx) { return ctx->tuned_sq > 0 ? ctx->tuned_sq : 24; }
 int default_copies(octmps_ctx *ctx) { return ctx->tuned_copies > 0 ? ctx->tuned_copies : 1; }
 
 void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid) {
@@ -458,60 +458,81 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     uint64_t per = std::max<uint64_t>(1, calib / (uint64_t)n);
     for (auto &s : cs) s.n_photons = std::min<uint64_t>(s.n_photons, per);
     std::vector<uint64_t> t((size_t)n * tally_words(p->num_depth_bins));
-    // Coordinate search: scatter quantum, then block size, then refill quantum, scatter
-    // quantum again (the two quanta interact: fewer parked lanes per scatter phase pair with
-    // more dead lanes per refill), then tally privatisation depth, then the entry-plane
-    // skip, each with the best values found so far for the others (25 calibration launches).
-    double best = -1;
-    octmps_batch_opts win = *opts;
-    win.block_size = 256;
-    win.refill_quantum = 16;
-    win.scatter_quantum = 40;
-    win.tally_copies = 1;
-    win.entry_skip = 0;
-    win.use_graph = 0;
-    auto trial = [&](octmps_batch_opts o) -> octmps_status {
+    // Coordinate search from the context defaults: scatter quantum, refill quantum, scatter
+    // quantum again (the two interact: fewer parked lanes per scatter phase pair with more
+    // dead lanes per refill), block size, tally privatisation depth, then the entry-plane
+    // skip. Launch options move a calibration launch by a few percent at most, so a candidate
+    // replaces the incumbent only when it is more than 1 % faster, and the final winner must
+    // beat the starting point again on a second measurement of both: the noise of a single
+    // launch no longer picks the options (18 calibration launches).
+    constexpr double kMargin = 1.01;
+    octmps_batch_opts start = *opts;
+    start.block_size = default_block(c);
+    start.refill_quantum = default_quantum(c);
+    start.scatter_quantum = default_scatter(c);
+    start.tally_copies = default_copies(c);
+    start.entry_skip = 0;
+    start.use_graph = 0;
+    auto measure = [&](const octmps_batch_opts &o, double &r) -> octmps_status {
         octmps_run_stats st;
         octmps_status s = run_scans(c, mesh_id, p, n, cs.data(), seed, &o, t.data(), &st, nullptr, 0, nullptr, 0);
-        if (s != OCTMPS_OK) return s;
-        double r = st.kernel_ms > 0 ? st.photons / (st.kernel_ms * 1e-3) : 0;
-        if (r > best) {
+        r = (s == OCTMPS_OK && st.kernel_ms > 0) ? st.photons / (st.kernel_ms * 1e-3) : 0;
+        return s;
+    };
+    octmps_batch_opts win = start;
+    double best = 0, r = 0;
+    octmps_status s;
+    if ((s = measure(win, best)) != OCTMPS_OK) return s;
+    const double start_rate = best;
+    auto trial = [&](octmps_batch_opts o) -> octmps_status {
+        octmps_status st = measure(o, r);
+        if (st == OCTMPS_OK && r > best * kMargin) {
             best = r;
             win = o;
         }
-        return OCTMPS_OK;
+        return st;
     };
-    octmps_status s;
-    const int scatter_quanta[] = {24, 28, 32, 36, 40, 44};
-    for (int sq : scatter_quanta) {
+    for (int sq : {20, 28, 32, 40}) {
         octmps_batch_opts o = win;
         o.scatter_quantum = sq;
-        if ((s = trial(o)) != OCTMPS_OK) return s;
+        if (sq != win.scatter_quantum && (s = trial(o)) != OCTMPS_OK) return s;
     }
-    for (int b : {64, 128, 256}) {
-        octmps_batch_opts o = win;
-        o.block_size = b;
-        if ((s = trial(o)) != OCTMPS_OK) return s;
-    }
-    for (int q : {8, 12, 16, 20, 24}) {
+    for (int q : {12, 16, 24}) {
         octmps_batch_opts o = win;
         o.refill_quantum = q;
-        if ((s = trial(o)) != OCTMPS_OK) return s;
+        if (q != win.refill_quantum && (s = trial(o)) != OCTMPS_OK) return s;
     }
-    for (int sq : scatter_quanta) {
+    for (int sq : {win.scatter_quantum - 4, win.scatter_quantum + 4}) {
         octmps_batch_opts o = win;
         o.scatter_quantum = sq;
-        if ((s = trial(o)) != OCTMPS_OK) return s;
+        if (sq >= 1 && (s = trial(o)) != OCTMPS_OK) return s;
     }
-    for (int k : {2, 4, 8}) {
+    for (int b : {128, 192}) {
+        octmps_batch_opts o = win;
+        o.block_size = b;
+        if (b != win.block_size && (s = trial(o)) != OCTMPS_OK) return s;
+    }
+    for (int k : {2, 4}) {
         octmps_batch_opts o = win;
         o.tally_copies = k;
-        if ((s = trial(o)) != OCTMPS_OK) return s;
+        if (k != win.tally_copies && (s = trial(o)) != OCTMPS_OK) return s;
     }
     for (int sk : {1, 2}) {
         octmps_batch_opts o = win;
         o.entry_skip = sk;
         if ((s = trial(o)) != OCTMPS_OK) return s;
+    }
+    if (std::memcmp(&win, &start, sizeof win) != 0) {
+        // Second measurement of both: keep the winner only if it still leads.
+        double r_win = 0, r_start = 0;
+        if ((s = measure(win, r_win)) != OCTMPS_OK || (s = measure(start, r_start)) != OCTMPS_OK) return s;
+        const double w = 0.5 * (best + r_win), s0 = 0.5 * (start_rate + r_start);
+        if (!(w > s0 * kMargin)) {
+            win = start;
+            best = s0;
+        } else {
+            best = w;
+        }
     }
     opts->block_size = win.block_size;
     opts->refill_quantum = win.refill_quantum;

@@ -65,6 +65,7 @@ struct Req {
     double t_submit = 0, t_admit = 0, t_done = 0;
     uint64_t photons = 0;
     int set = -1;
+    int64_t slot_base = -1;  // first of the request's contiguous scan slots (while admitted)
     uint64_t tally_base = 0;
     std::vector<ScanState> ss;
 };
@@ -115,7 +116,8 @@ struct octmps_server {
     int grid = 0, block = 0, q = 0;
     uint64_t lanes = 0;
     int n_slots = 0;
-    std::vector<int> free_slots, free_sets;
+    std::vector<Range> free_slots;  // contiguous free scan-slot ranges
+    std::vector<int> free_sets;
     std::vector<Range> free_tally;
     uint64_t tally_cap_words = 0;
     bool have_saved = false;
@@ -135,7 +137,34 @@ struct octmps_server {
 namespace octmps {
 namespace {
 
-void free_tally(octmps_server *s, uint64_t off, uint64_t words);
+// First-fit allocation from a list of free ranges sorted by offset.
+bool range_alloc(std::vector<Range> &free, uint64_t n, uint64_t &off) {
+    for (size_t i = 0; i < free.size(); ++i) {
+        Range &r = free[i];
+        if (r.len >= n) {
+            off = r.off;
+            r.off += n;
+            r.len -= n;
+            if (r.len == 0) free.erase(free.begin() + i);
+            return true;
+        }
+    }
+    return false;
+}
+
+// Returns a range to the list, merging it with its neighbours.
+void range_free(std::vector<Range> &free, uint64_t off, uint64_t n) {
+    auto it = std::lower_bound(free.begin(), free.end(), off, [](const Range &r, uint64_t o) { return r.off < o; });
+    it = free.insert(it, Range{off, n});
+    if (it + 1 != free.end() && it->off + it->len == (it + 1)->off) {
+        it->len += (it + 1)->len;
+        free.erase(it + 1);
+    }
+    if (it != free.begin() && (it - 1)->off + (it - 1)->len == it->off) {
+        (it - 1)->len += it->len;
+        free.erase(it);
+    }
+}
 
 // Grows the device tally pool to hold at least `words` more (a new allocation; the live
 // tallies are copied over). Called between slices, from the scheduler thread.
@@ -156,41 +185,18 @@ bool grow_tallies(octmps_server *s, uint64_t words) {
     s->tallies.p = np;
     s->tallies.cap = ncap * 8;
     s->tally_cap_words = ncap;
-    free_tally(s, old, ncap - old);
+    range_free(s->free_tally, old, ncap - old);
     return true;
 }
 
 bool alloc_tally(octmps_server *s, uint64_t words, uint64_t &off) {
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1 && !grow_tallies(s, words)) return false;
-        for (size_t i = 0; i < s->free_tally.size(); ++i) {
-            Range &r = s->free_tally[i];
-            if (r.len >= words) {
-                off = r.off;
-                r.off += words;
-                r.len -= words;
-                if (r.len == 0) s->free_tally.erase(s->free_tally.begin() + i);
-                return true;
-            }
-        }
-    }
-    return false;
-}
-
-void free_tally(octmps_server *s, uint64_t off, uint64_t words) {
-    s->free_tally.push_back({off, words});
-    std::sort(s->free_tally.begin(), s->free_tally.end(), [](const Range &a, const Range &b) { return a.off < b.off; });
-    std::vector<Range> m;
-    for (const Range &r : s->free_tally) {
-        if (!m.empty() && m.back().off + m.back().len == r.off) m.back().len += r.len;
-        else m.push_back(r);
-    }
-    s->free_tally.swap(m);
+    return range_alloc(s->free_tally, words, off) ||
+           (grow_tallies(s, words) && range_alloc(s->free_tally, words, off));
 }
 
 void release(octmps_server *s, Req *r) {
-    for (ScanState &x : r->ss)
-        if (x.slot >= 0) s->free_slots.push_back(x.slot);
+    if (r->slot_base >= 0) range_free(s->free_slots, (uint64_t)r->slot_base, r->scans.size());
+    r->slot_base = -1;
     if (r->set >= 0) {
         // Smallest id first: ids stay dense (the slice maps the sets in flight to LDS
         // slots whatever their ids).
@@ -198,7 +204,7 @@ void release(octmps_server *s, Req *r) {
         std::sort(s->free_sets.begin(), s->free_sets.end(), std::greater<int>());
     }
     if (r->admitted) {
-        free_tally(s, r->tally_base, r->words * r->scans.size());
+        range_free(s->free_tally, r->tally_base, r->words * r->scans.size());
         MeshUse &u = s->mesh_use[r->mesh_id];
         u.users--;
         u.last_use = now_ms();
@@ -225,15 +231,18 @@ octmps_status stage(octmps_server *s, Req *r) {
                             R.size() * sizeof(DevRegion), hipMemcpyHostToDevice, s->stream)) ||
         (e = hipMemcpyAsync((DevSet *)s->sets.p + r->set, &S, sizeof S, hipMemcpyHostToDevice, s->stream)))
         return hip_fail(e, "serve stage");
-    for (size_t i = 0; i < r->scans.size(); ++i) {
-        DevScan d;
-        st = make_scan(ctx, me, &r->p, r->scans[i], r->seed, r->set, r->tally_base + i * r->words, d);
+    // The request's scan slots are one contiguous range: one copy of their records and one
+    // fill of their completion counters.
+    const size_t n = r->scans.size();
+    std::vector<DevScan> d(n);
+    for (size_t i = 0; i < n; ++i) {
+        st = make_scan(ctx, me, &r->p, r->scans[i], r->seed, r->set, r->tally_base + i * r->words, d[i]);
         if (st != OCTMPS_OK) return st;
-        int slot = r->ss[i].slot;
-        if ((e = hipMemcpyAsync((DevScan *)s->slots.p + slot, &d, sizeof d, hipMemcpyHostToDevice, s->stream)) ||
-            (e = launch_fill_u64((uint64_t *)s->slot_done.p + slot, 1, 0, s->stream)))
-            return hip_fail(e, "serve stage");
     }
+    if ((e = hipMemcpyAsync((DevScan *)s->slots.p + r->slot_base, d.data(), n * sizeof(DevScan),
+                            hipMemcpyHostToDevice, s->stream)) ||
+        (e = launch_fill_u64((uint64_t *)s->slot_done.p + r->slot_base, n, 0, s->stream)))
+        return hip_fail(e, "serve stage");
     if ((e = launch_fill_u64((uint64_t *)s->tallies.p + r->tally_base, r->words * r->scans.size(), 0, s->stream)))
         return hip_fail(e, "serve stage");
     return OCTMPS_OK;
@@ -264,7 +273,7 @@ void admit(octmps_server *s, std::vector<Req *> &staged) {
             continue;
         }
         const size_t need_slots = r->scans.size();
-        if (s->free_slots.size() < need_slots || s->free_sets.empty()) break;
+        if (s->free_sets.empty()) break;
         const uint64_t charge = r->words * need_slots * 8 + request_state_bytes(need_slots);
         const bool counted_mesh = s->mesh_use.count(r->mesh_id) != 0;
         const uint64_t mesh_bytes = counted_mesh ? 0 : r->me->device_bytes();
@@ -280,8 +289,11 @@ void admit(octmps_server *s, std::vector<Req *> &staged) {
             s->mesh_use.erase(lru);
         }
         if (s->used + charge + mesh_bytes > s->budget) break;  // waits for admitted requests to end
+        uint64_t slot_off;
+        if (!range_alloc(s->free_slots, need_slots, slot_off)) break;  // waits for slots to free up
         uint64_t off;
         if (!alloc_tally(s, r->words * need_slots, off)) {
+            range_free(s->free_slots, slot_off, need_slots);
             if (s->active.empty()) {  // the device cannot hold its tallies at all
                 s->pending.pop_front();
                 end_unadmitted(s, r, OCTMPS_E_NOMEM);
@@ -299,10 +311,8 @@ void admit(octmps_server *s, std::vector<Req *> &staged) {
         r->admitted = true;
         r->set = s->free_sets.back();
         s->free_sets.pop_back();
-        for (size_t i = 0; i < need_slots; ++i) {
-            r->ss[i].slot = s->free_slots.back();
-            s->free_slots.pop_back();
-        }
+        r->slot_base = (int64_t)slot_off;
+        for (size_t i = 0; i < need_slots; ++i) r->ss[i].slot = (int)(slot_off + i);
         r->t_admit = now_ms() - s->t0;
         s->pending.pop_front();
         s->active.push_back(r);
@@ -626,7 +636,7 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
         octmps_serve_close(s);
         return fail(OCTMPS_E_NOMEM, std::string("server allocation: ") + hipGetErrorString(e));
     }
-    for (int i = s->n_slots - 1; i >= 0; --i) s->free_slots.push_back(i);
+    s->free_slots.push_back({0, (uint64_t)s->n_slots});
     for (int i = kMaxSets - 1; i >= 0; --i) s->free_sets.push_back(i);
     s->free_tally.push_back({0, s->tally_cap_words});
     s->th = std::thread(scheduler, s);
