@@ -76,7 +76,7 @@ def test_struct_layouts_match_the_header(tmp_path):
 
 
 def test_integration_example_compiles(tmp_path):
-    # The C++ binding example in INTEGRATION.md, wrapped in a main that declares the names it
+    # The C++ binding example of the integration guide, wrapped in a main that declares the names it
     # takes from the reference's host code, must compile against include/octmps.h.
     import re
     import subprocess

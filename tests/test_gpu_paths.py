@@ -362,7 +362,7 @@ def test_cli_runs_every_run_of_a_multi_run_opt(tmp_path, ctx):
 
 
 def test_integration_python_example_runs(tmp_path, ctx, monkeypatch):
-    # The Python example of INTEGRATION.md, run as written on a small two-layer case: its
+    # The Python example of the integration guide, run as written on a small two-layer case: its
     # A-scan equals the library's own plain run of the same inputs.
     import re
 
