@@ -266,11 +266,19 @@ typedef struct {
     int32_t tally_copies;    /* tally privatisation depth: private copies of every A-scan's
                                 tally (workgroup w adds into copy w mod k), folded into one
                                 after the launch; 1..16, 0 = autotuned/default (1) */
-    int32_t entry_skip;      /* a photon that crossed into a tet through face k does not fetch
-                                plane k: 0 = auto (on when the mesh's 128-byte tet records fit
-                                the 256 MB Infinity Cache), 1 = on, 2 = off. Results are the
-                                same bits either way. */
+    int32_t entry_skip;      /* per-lane record fetch only: a photon that crossed into a tet
+                                through face k does not fetch plane k: 0 = auto (on when the
+                                mesh's 128-byte tet records fit the 256 MB Infinity Cache),
+                                1 = on, 2 = off. Results are the same bits either way. */
     uint64_t max_steps_per_photon; /* 0 = default guard (1<<22) */
+    int32_t record_fetch;    /* how a wave reads the tet records of its crossing photons:
+                                1 = each lane gathers its own record (five 16-byte loads);
+                                2 = cooperatively: four lanes read the four face planes of one
+                                record together into LDS (one L1 cycle instead of four), and a
+                                record stays staged while its photon stays in the tet;
+                                0 = auto (2 when its launch holds as many waves per CU as 1,
+                                which needs 256-lane blocks). Results are the same bits. */
+    int32_t reserved0;       /* 0 */
 } octmps_batch_opts;
 
 /* Plain path: one A-scan per launch sequence, synchronous, tallies copied back
@@ -314,8 +322,9 @@ octmps_status octmps_run_bscan_logged(octmps_ctx *ctx, int32_t mesh_id, const oc
  * defaults, 18 launches: a candidate must be more than 1 % faster to win, and the winner must
  * lead the starting point again on a second measurement of both);
  * writes the winner into *opts (scatter_quantum, block_size, refill_quantum, tally_copies,
- * entry_skip), makes the first four the context's default (entry_skip stays per launch, since
- * its best value depends on the mesh), and returns its photons/s in *rate. */
+ * entry_skip, record_fetch), makes the first four the context's default (entry_skip and
+ * record_fetch stay per launch, since their best values depend on the mesh), and returns its
+ * photons/s in *rate. */
 octmps_status octmps_autotune(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p,
                               int32_t n_scans, const octmps_ascan *scans, uint64_t seed,
                               uint64_t calib_photons, octmps_batch_opts *opts, double *rate);

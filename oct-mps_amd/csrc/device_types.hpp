@@ -140,6 +140,8 @@ struct KArgs {
     const TetRec *tets;
     uint32_t tets_bytes;        // bytes of resident records (<= (2^25 - 1) * 128 < 2^32)
     uint32_t skip_entry;        // 1: the plane of a tet's entry face is not fetched
+    uint32_t coop_fetch;        // 1 (single-set): tet records fetched cooperatively into LDS
+    uint32_t n_regions;         // coop_fetch: regions of the set (its LDS region table)
     uint32_t sets_in_lds;       // multi-set launch: every lane's set is in the LDS map, regions < kLdsSetRegions
     const uint8_t *set_slot;    // sets_in_lds: LDS slot of each parameter set id [kMaxServeSets]
     const int32_t *lds_set_ids; // sets_in_lds: parameter set id held in each LDS slot [kLdsSets]
@@ -181,7 +183,8 @@ hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t stre
 hipError_t launch_fold_tallies(uint64_t *p, uint64_t n_scans, int bins, uint32_t copies, uint64_t stride,
                                hipStream_t stream);
 hipError_t launch_clear_saved(SavedLane *p, uint64_t n, hipStream_t stream);
-int transport_blocks_per_cu(int block, bool multi, bool log);
+// coop_regions > 0: the cooperative-fetch variant with that many regions in its LDS table.
+int transport_blocks_per_cu(int block, bool multi, bool log, uint32_t coop_regions);
 hipError_t launch_probe_rng(const uint32_t *mult, uint32_t n_triplets, uint64_t seed, uint64_t salt,
                             uint64_t photon0, int n_photons, int n_draws, float *out,
                             hipStream_t stream);

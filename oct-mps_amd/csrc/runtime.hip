@@ -149,8 +149,14 @@ int default_quantum(octmps_ctx *ctx) { return ctx->tuned_q > 0 ? ctx->tuned_q : 
 int default_scatter(octmps_ctx *ctx) { return ctx->tuned_sq > 0 ? ctx->tuned_sq : 24; }
 int default_copies(octmps_ctx *ctx) { return ctx->tuned_copies > 0 ? ctx->tuned_copies : 1; }
 
-void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid) {
-    int occ = transport_blocks_per_cu(block, multi, false);
+// The cooperative record fetch by default: when its LDS lets a CU hold as many waves as the
+// per-lane variants (256-lane blocks and a set with few regions).
+bool coop_fetch_fits(int block, uint32_t n_regions) {
+    return transport_blocks_per_cu(block, false, false, n_regions) >= transport_blocks_per_cu(block, false, false, 0);
+}
+
+void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid, uint32_t coop_regions) {
+    int occ = transport_blocks_per_cu(block, multi, false, coop_regions);
     if (occ <= 0) occ = 1;
     int b = blocks_per_cu > 0 ? std::min(blocks_per_cu, occ) : occ;
     grid = std::max(1, b * ctx->n_cu);
@@ -232,8 +238,11 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     if (copies > 16) return fail(OCTMPS_E_INVALID, "tally_copies must be 1..16");
     const int skip_opt = opts ? opts->entry_skip : 0;
     if (skip_opt < 0 || skip_opt > 2) return fail(OCTMPS_E_INVALID, "entry_skip must be 0, 1 or 2");
+    const int fetch_opt = opts ? opts->record_fetch : 0;
+    if (fetch_opt < 0 || fetch_opt > 2) return fail(OCTMPS_E_INVALID, "record_fetch must be 0, 1 or 2");
+    const bool coop = fetch_opt == 2 || (fetch_opt == 0 && coop_fetch_fits(block, p->n_regions));
     int grid = 0;
-    grid_for(ctx, block, opts ? opts->blocks_per_cu : 0, false, grid);
+    grid_for(ctx, block, opts ? opts->blocks_per_cu : 0, false, grid, coop ? p->n_regions : 0);
     hipError_t e;
     hipStream_t sm = ctx->stream;
     RunBuffers D;
@@ -282,6 +291,8 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
     A.tets = ctx->pool_tets;
     A.tets_bytes = (uint32_t)(ctx->pool_cap * (int64_t)sizeof(TetRec));
     A.skip_entry = skip_opt == 1 || (skip_opt == 0 && entry_skip_fits(me.n_tets));
+    A.coop_fetch = coop;
+    A.n_regions = p->n_regions;
     A.regions = (const DevRegion *)D.regions;
     A.sets = (const DevSet *)D.sets;
     A.slots = (const DevScan *)D.slots;
@@ -460,8 +471,8 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     std::vector<uint64_t> t((size_t)n * tally_words(p->num_depth_bins));
     // Coordinate search from the context defaults: scatter quantum, refill quantum, scatter
     // quantum again (the two interact: fewer parked lanes per scatter phase pair with more
-    // dead lanes per refill), block size, tally privatisation depth, then the entry-plane
-    // skip. Launch options move a calibration launch by a few percent at most, so a candidate
+    // dead lanes per refill), block size, tally privatisation depth, then the record path
+    // (per-lane gathers with or without the entry-plane skip). Launch options move a calibration launch by a few percent at most, so a candidate
     // replaces the incumbent only when it is more than 1 % faster, and the final winner must
     // beat the starting point again on a second measurement of both: the noise of a single
     // launch no longer picks the options (18 calibration launches).
@@ -472,6 +483,7 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     start.scatter_quantum = default_scatter(c);
     start.tally_copies = default_copies(c);
     start.entry_skip = 0;
+    start.record_fetch = 0;
     start.use_graph = 0;
     auto measure = [&](const octmps_batch_opts &o, double &r) -> octmps_status {
         octmps_run_stats st;
@@ -517,8 +529,11 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
         o.tally_copies = k;
         if (k != win.tally_copies && (s = trial(o)) != OCTMPS_OK) return s;
     }
+    // Record path: the per-lane gathers with and without the entry-plane skip, against the
+    // default (the cooperative fetch where it fits).
     for (int sk : {1, 2}) {
         octmps_batch_opts o = win;
+        o.record_fetch = 1;
         o.entry_skip = sk;
         if ((s = trial(o)) != OCTMPS_OK) return s;
     }
@@ -539,6 +554,7 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     opts->scatter_quantum = win.scatter_quantum;
     opts->tally_copies = win.tally_copies;
     opts->entry_skip = win.entry_skip;
+    opts->record_fetch = win.record_fetch;
     c->tuned_block = win.block_size;
     c->tuned_q = win.refill_quantum;
     c->tuned_sq = win.scatter_quantum;

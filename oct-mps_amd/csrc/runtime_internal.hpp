@@ -96,7 +96,7 @@ struct octmps_ctx {
 namespace octmps {
 octmps_status make_scan(octmps_ctx *ctx, const MeshEntry &me, const octmps_params *p, const octmps_ascan &s,
                         uint64_t seed, int set, uint64_t tally_off, DevScan &d);
-void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid);
+void grid_for(octmps_ctx *ctx, int block, int blocks_per_cu, bool multi, int &grid, uint32_t coop_regions = 0);
 // Runs A-scans to completion in one persistent launch; device buffers come from the context's
 // work buffers, or from `arena` (arena_bytes) when it is set (runtime.hip).
 octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p, int32_t n,

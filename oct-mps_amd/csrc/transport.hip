@@ -39,6 +39,8 @@ constexpr uint32_t F_ALIVE = 1u, F_FST = 2u, F_NBS = 4u, F_CNBS = 8u, F_PEND = 1
 constexpr uint32_t kFaceShift = 6, F_FACE = 3u << kFaceShift;
 // Entry face of the current tet (0: none, k + 1: the photon crossed into it through face k).
 constexpr uint32_t kEntryShift = 8, F_ENTRY = 7u << kEntryShift;
+// Cooperative fetch: the lane's LDS stage holds the planes and links of its current tet.
+constexpr uint32_t F_STAGED = 1u << 11;
 
 typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 // Buffer-resource word 3 for raw (untyped) loads on CDNA: 32-bit data format.
@@ -46,6 +48,10 @@ constexpr int kBufferWord3 = 0x00020000;
 // A record offset past every resident record (tets_bytes <= 2^32 - 128): a raw buffer load
 // there is out of range, so it makes no memory request and returns zeros.
 constexpr uint32_t kOutOfRange = 0xFFFFFF80u;
+// Cooperative fetch: a wave's LDS stage holds four 64-lane plane loads at float4 offsets 0,
+// 65, 130 and 195 (the one-float4 gaps spread consecutive owners over the LDS banks), then
+// the links of lane l at kStageLinks + l.
+constexpr uint32_t kStageLinks = 260, kStageF4 = kStageLinks + 64;
 
 // __uint2float_rz(u) / 2^32: keep the top 24 significant bits, then an exact scale.
 __device__ __forceinline__ float unit_rz24(uint32_t u) {
@@ -242,8 +248,20 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
 // a lane then carries its set's LDS slot (A.set_slot) in place of the set id. Otherwise
 // each lane reads its set's records from memory, a per-lane gather in the parked phase
 // that made serving 25 % slower than a single-set launch.
-template <bool MULTI, bool LOG, bool SKIP, bool SETS_LDS>
+//
+// COOP (single-set only): tet records are fetched cooperatively into a per-wave LDS stage.
+// A 16-byte vector load costs the L1 one lookup cycle per lane when the 64 lanes of an
+// instruction hit 64 different lines, while four lanes reading the four 16-byte quarters of
+// one 64-byte piece share one cycle. So four DMA loads (buffer_load ... lds) give quad q of
+// instruction j the four planes of the tet of its lane j (a DPP broadcast of that lane's
+// record offset), and one more gives each lane its tet's links: two L1 cycles per record
+// instead of five. A lane's stage keeps its tet (F_STAGED) until the photon moves to
+// another tet, so a photon that scattered, or was reflected, in its tet reads it again from
+// LDS, and a photon that moved fetches its new tet at the end of the iteration, ahead of
+// the crossing phase that needs it. Parked lanes read their exit face from the stage.
+template <bool MULTI, bool LOG, bool SKIP, bool SETS_LDS, bool COOP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) transport_kernel(KArgs A) {
+    static_assert(!(COOP && (MULTI || SKIP)), "the cooperative fetch is a single-set variant without the skip");
     const uint32_t lane = __lane_id();
     const uint64_t gl = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 
@@ -271,11 +289,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     // its steps in layered tissue. MULTI with SETS_LDS keeps the first kLdsSets sets and
     // their region tables in LDS; MULTI without it reads each lane's set from memory.
     const DevSet S0 = A.sets[0];
-    constexpr int kRegionSlots = MULTI ? (SETS_LDS ? kLdsSets * kLdsSetRegions : 1) : kMaxRegions;
+    constexpr int kRegionSlots = MULTI ? (SETS_LDS ? kLdsSets * kLdsSetRegions : 1) : (COOP ? 1 : kMaxRegions);
     __shared__ DevRegion lds_regions[kRegionSlots];
     __shared__ DevSet lds_sets[MULTI && SETS_LDS ? kLdsSets : 1];
+    // COOP: the stages of the (at most four) waves in their own LDS object, so that the
+    // compiler does not make reads of the other LDS data wait for the stage loads; a probe
+    // position per lane and the set's region table in dynamic LDS sized by the launch.
+    __shared__ float4 lds_stage[COOP ? 4 * kStageF4 : 1];
+    extern __shared__ float4 lds_dyn[];
+    DevRegion *const coop_regions = reinterpret_cast<DevRegion *>(lds_dyn + blockDim.x);
     if (!MULTI) {
-        for (int i = threadIdx.x; i < kMaxRegions; i += blockDim.x) lds_regions[i] = A.regions[S0.region_base + i];
+        if (COOP)
+            for (int i = threadIdx.x; i < (int)A.n_regions; i += blockDim.x) coop_regions[i] = A.regions[S0.region_base + i];
+        else
+            for (int i = threadIdx.x; i < kMaxRegions; i += blockDim.x) lds_regions[i] = A.regions[S0.region_base + i];
         __syncthreads();
     } else if (SETS_LDS) {
         for (int i = threadIdx.x; i < kLdsSets * kLdsSetRegions; i += blockDim.x)
@@ -287,11 +314,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     // A-scan, written at launch, and the exit face of a lane parked at an interface (normal
     // and packed neighbour word), written when it parks, so that the parked phase reads them
     // from LDS instead of making a dependent global round trip.
-    __shared__ float4 lds_probe[256], lds_face[256];
+    // COOP reads the face from the lane's stage and keeps the probe in dynamic LDS.
+    __shared__ float4 lds_probe_s[COOP ? 1 : 256], lds_face[COOP ? 1 : 256];
+    auto lds_probe_at = [&]() -> float4 & {
+        return COOP ? lds_dyn[threadIdx.x] : lds_probe_s[threadIdx.x];
+    };
+    // This lane's four planes in the stage: owner l was fetched by instruction l & 3, quad
+    // l >> 2, at 65 (l & 3) + 4 (l >> 2).
+    const float4 *const own_planes = lds_stage + (threadIdx.x >> 6) * kStageF4 + 65u * (lane & 3u) + (lane & ~3u);
+    const uint4 *const own_links = reinterpret_cast<const uint4 *>(lds_stage + (threadIdx.x >> 6) * kStageF4 + kStageLinks + lane);
     auto region_rec = [&](int r) -> const DevRegion & {
         // The region table of set s starts at s * kMaxRegions (DevSet::region_base).
         return MULTI ? (SETS_LDS ? lds_regions[set * kLdsSetRegions + r] : A.regions[set * kMaxRegions + r])
-                     : lds_regions[r];
+                     : (COOP ? coop_regions[r] : lds_regions[r]);
     };
     auto load_region = [&](int r) {
         const DevRegion &R = region_rec(r);
@@ -315,7 +350,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             if (MULTI && SETS_LDS) set = A.set_slot[set];
             load_region(region);
             const DevScan &sc = A.slots[slot];
-            lds_probe[threadIdx.x] = make_float4(sc.px, sc.py, sc.pz, 0.0f);
+            lds_probe_at() = make_float4(sc.px, sc.py, sc.pz, 0.0f);
             if (flags & F_IFACE) {
                 const uint32_t f = (flags & F_FACE) >> kFaceShift;
                 const float4 pl = A.tets[tet].plane[f];
@@ -332,6 +367,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     const uint32_t q = A.refill_q < 1u ? 1u : A.refill_q;
     const __amdgpu_buffer_rsrc_t tets_rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *)A.tets, (short)0, (int)A.tets_bytes, kBufferWord3);
+
+    // Cooperative fetch (COOP; call with the whole wave): stages the planes and links of the
+    // tets of the lanes with `want` set. Instruction j: quad q fetches the planes of the tet of
+    // its lane j (owner 4q + j, a DPP broadcast within the quad), lane l the plane l & 3; then
+    // every wanting lane fetches its own links. The loads complete into LDS (vmcnt).
+    auto coop_fetch = [&](bool want) {
+        if (__ballot(want) == 0) return;
+        const int own = want ? (int)((uint32_t)tet << 7) : -1;
+        const int r0 = __builtin_amdgcn_mov_dpp(own, 0x00, 0xF, 0xF, false);
+        const int r1 = __builtin_amdgcn_mov_dpp(own, 0x55, 0xF, 0xF, false);
+        const int r2 = __builtin_amdgcn_mov_dpp(own, 0xAA, 0xF, 0xF, false);
+        const int r3 = __builtin_amdgcn_mov_dpp(own, 0xFF, 0xF, 0xF, false);
+        const uint32_t quarter = (lane & 3u) << 4;
+        // The wave's stage (wave-uniform, so the LDS base can go in M0).
+        float4 *stage = lds_stage + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * kStageF4;
+        typedef __attribute__((address_space(3))) void *lds_ptr;
+        if (r0 != -1)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage), 16, (uint32_t)r0 + quarter, 0, 0, 0);
+        if (r1 != -1)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage + 65), 16, (uint32_t)r1 + quarter, 0, 0, 0);
+        if (r2 != -1)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage + 130), 16, (uint32_t)r2 + quarter, 0, 0, 0);
+        if (r3 != -1)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage + 195), 16, (uint32_t)r3 + quarter, 0, 0, 0);
+        if (want) {
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage + kStageLinks), 16, (uint32_t)own + 64u, 0, 0, 0);
+            flags |= F_STAGED;
+        }
+    };
 
     for (;;) {
         // ---- refill dead lanes (wave-uniform decision)
@@ -369,7 +433,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                         sR = derive_state(seeding_state(h, 1, b1), b1, aR);
                         sS = derive_state(seeding_state(h, 2, b2), b2, aS);
                         x = sc.px; y = sc.py; z = sc.pz;
-                        lds_probe[threadIdx.x] = make_float4(sc.px, sc.py, sc.pz, 0.0f);
+                        lds_probe_at() = make_float4(sc.px, sc.py, sc.pz, 0.0f);
                         ux = 0.0f; uy = 0.0f; uz = 1.0f;
                         w = sc.w0;
                         // The first step length (General stream) is drawn at launch; the
@@ -455,6 +519,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             return true;
         };
 
+        if (COOP) {
+            // Lanes about to cross whose stage does not hold their tet yet (most were
+            // fetched ahead, at the end of the previous iteration).
+            coop_fetch(ready && !(steps + 1u > max_steps) && !(flags & F_STAGED));
+        }
+
         if (!ready) {
             // parked or dead lane: nothing to do in the crossing phase
         } else if (++steps > max_steps) {
@@ -485,10 +555,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                 return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
                                    __uint_as_float(v.w));
             };
-            const float4 p0 = plane_at(0), p1 = plane_at(1), p2 = plane_at(2), p3 = plane_at(3);
-            const u4v lv = __builtin_amdgcn_raw_buffer_load_b128(tets_rsrc, rec + 64u, 0, 0);
-            const uint4 L = make_uint4(lv.x, lv.y, lv.z, lv.w);
-            // Keep the five loads together ahead of the face tests (one memory round trip).
+            float4 p0, p1, p2, p3;
+            uint4 L;
+            if (COOP) {
+                // The stage loads write LDS when they complete: wait for all of the wave's
+                // vector memory operations (s_waitcnt vmcnt(0)) before reading it.
+                __builtin_amdgcn_s_waitcnt(0x0F70);
+                asm volatile("" ::: "memory");
+                L = *own_links;
+                p0 = own_planes[0];
+                p1 = own_planes[1];
+                p2 = own_planes[2];
+                p3 = own_planes[3];
+            } else {
+                p0 = plane_at(0);
+                p1 = plane_at(1);
+                p2 = plane_at(2);
+                p3 = plane_at(3);
+                const u4v lv = __builtin_amdgcn_raw_buffer_load_b128(tets_rsrc, rec + 64u, 0, 0);
+                L = make_uint4(lv.x, lv.y, lv.z, lv.w);
+            }
+            // Keep the loads together ahead of the face tests (one memory round trip).
             __builtin_amdgcn_sched_barrier(0);
             const float c0 = fmaf(p0.z, uz, fmaf(p0.y, uy, p0.x * ux));
             const float c1 = fmaf(p1.z, uz, fmaf(p1.y, uy, p1.x * ux));
@@ -539,7 +626,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                     // which it is exactly when the new MWC word is 0.
                     if (mwc_next(sR, aR) != 0u) {
                         tet = (int)(pk & kLinkMask);
-                        flags |= (((pk >> kBackShift) & 3u) + 1u) << kEntryShift;
+                        flags = (flags & ~F_STAGED) | ((((pk >> kBackShift) & 3u) + 1u) << kEntryShift);
                     } else {
                         const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
                         const float two_ca1 = -2.0f * mincos;
@@ -549,8 +636,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                         uz = fmaf(two_ca1, pf.z, uz);
                     }
                 } else {
-                    const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
-                    lds_face[threadIdx.x] = make_float4(pf.x, pf.y, pf.z, __uint_as_float(pk));
+                    // COOP: the face's normal and neighbour word stay in the lane's stage.
+                    if (!COOP) {
+                        const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
+                        lds_face[threadIdx.x] = make_float4(pf.x, pf.y, pf.z, __uint_as_float(pk));
+                    }
                     flags |= F_IFACE | ((uint32_t)imin << kFaceShift);
                 }
             } else {
@@ -568,12 +658,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             const bool parked = (flags & (F_PEND | F_IFACE)) != 0;
             if (flags & F_IFACE) {
                 // ---- FastReflectTransmit, src/octmps_kernel.cuh:590-805
+                const uint32_t fidx = (flags & F_FACE) >> kFaceShift;
                 flags &= ~(F_IFACE | F_FACE);
                 // The exit face's normal and packed neighbour word (pool tet id, all ones at
-                // the mesh boundary, and its region), stashed when the lane parked.
-                const float4 fr = lds_face[threadIdx.x];
-                const float3 pf = make_float3(fr.x, fr.y, fr.z);
-                const uint32_t pk = __float_as_uint(fr.w);
+                // the mesh boundary, and its region): in the stage (COOP), or stashed when the
+                // lane parked.
+                float3 pf;
+                uint32_t pk;
+                if (COOP) {
+                    const float4 fr = own_planes[fidx];
+                    const uint4 lk = *own_links;
+                    pf = make_float3(fr.x, fr.y, fr.z);
+                    pk = fidx == 0 ? lk.x : (fidx == 1 ? lk.y : (fidx == 2 ? lk.z : lk.w));
+                } else {
+                    const float4 fr = lds_face[threadIdx.x];
+                    pf = make_float3(fr.x, fr.y, fr.z);
+                    pk = __float_as_uint(fr.w);
+                }
                 const int nxt = (pk & kLinkMask) == kLinkMask ? -1 : (int)(pk & kLinkMask);
                 const uint32_t field = pk >> kRegionShift;
                 // An escaped link carries no region: read it from the neighbour's record.
@@ -598,7 +699,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                 if (nxt < 0 && R < 1.0f) {
                     if (uz < 0.0f) {
                         bool flag = false;
-                        const float4 probe = lds_probe[threadIdx.x];
+                        const float4 probe = lds_probe_at();
                         const float dx = x - probe.x, dy = y - probe.y;
                         if (fmaf(dx, dx, dy * dy) < S.radius2 && -uz > S.cos_angle) {
                             const int tb = S.type_bias;
@@ -624,6 +725,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                     uz = rz;
                 } else if (rnd > R) {
                     tet = nxt;
+                    flags &= ~F_STAGED;
                     if (field != kRegionEscape) flags |= (((pk >> kBackShift) & 3u) + 1u) << kEntryShift;
                     if (nr != region) {
                         region = nr;
@@ -663,7 +765,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                         atmp = S.bias_a;
                         first = true;
                     }
-                    const float4 probe = lds_probe[threadIdx.x];
+                    const float4 probe = lds_probe_at();
                     float vx = probe.x - x, vy = probe.y - y, vz = probe.z - z;
                     const float len2 = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
                     const float rl = rsqrtf(len2);
@@ -737,6 +839,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                 A.plog[id] = r;
             }
         }
+        // COOP: a photon that moved into another tet (crossing a face, or an interface in the
+        // parked phase, or a restored continuation photon) and keeps crossing fetches its tet
+        // here, ahead of the next crossing phase. Fetching at the end of the crossing phase
+        // instead made the parked phase wait for those loads (its reads of the stage, and of
+        // spilled registers, wait for all of the wave's vector memory operations).
+        if (COOP) coop_fetch((flags & (F_ALIVE | F_PEND | F_IFACE | F_STAGED)) == F_ALIVE);
     }
 
     if (SLICES && A.slice_mode) {
@@ -809,9 +917,17 @@ __global__ void probe_sampler_kernel(int kind, float p, int n, const float *u, f
 namespace {
 // The single-set variants come with and without the entry skip; the multi-set (serving)
 // variant never skips (its register budget is full: with the skip it spills 14 registers).
+// Dynamic LDS of the cooperative-fetch variant: a probe per lane, then the region table.
+size_t coop_lds_bytes(int block, uint32_t n_regions) {
+    return (size_t)block * sizeof(float4) + (size_t)n_regions * sizeof(DevRegion);
+}
+
 template <bool LOG>
 void launch_single(const KArgs &a, int grid, int block, hipStream_t stream) {
-    if (a.skip_entry) hipLaunchKernelGGL((transport_kernel<false, LOG, true, false>), dim3(grid), dim3(block), 0, stream, a);
+    if (a.coop_fetch)
+        hipLaunchKernelGGL((transport_kernel<false, LOG, false, false, true>), dim3(grid), dim3(block),
+                           coop_lds_bytes(block, a.n_regions), stream, a);
+    else if (a.skip_entry) hipLaunchKernelGGL((transport_kernel<false, LOG, true, false>), dim3(grid), dim3(block), 0, stream, a);
     else hipLaunchKernelGGL((transport_kernel<false, LOG, false, false>), dim3(grid), dim3(block), 0, stream, a);
 }
 
@@ -831,10 +947,14 @@ int occupancy_multi(int block) {
     return a < b ? a : b;
 }
 
-// Blocks per CU of the tighter of the two single-set variants.
+// Blocks per CU of the tighter of the two single-set variants, or of the cooperative one
+// (coop_regions > 0: its LDS holds that many regions).
 template <bool LOG>
-int occupancy_single(int block) {
+int occupancy_single(int block, uint32_t coop_regions) {
     int a = 0, b = 0;
+    if (coop_regions)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, transport_kernel<false, LOG, false, false, true>, block,
+                                                            coop_lds_bytes(block, coop_regions)) == hipSuccess ? a : 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, transport_kernel<false, LOG, true, false>, block, 0) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, transport_kernel<false, LOG, false, false>, block, 0) != hipSuccess)
         return 0;
@@ -853,9 +973,9 @@ hipError_t launch_transport(const KArgs &a, int grid, int block, bool multi, boo
     return hipGetLastError();
 }
 
-int transport_blocks_per_cu(int block, bool multi, bool log) {
+int transport_blocks_per_cu(int block, bool multi, bool log, uint32_t coop_regions) {
     if (multi) return log ? occupancy_multi<true>(block) : occupancy_multi<false>(block);
-    return log ? occupancy_single<true>(block) : occupancy_single<false>(block);
+    return log ? occupancy_single<true>(block, coop_regions) : occupancy_single<false>(block, coop_regions);
 }
 
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t stream) {

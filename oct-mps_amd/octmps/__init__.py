@@ -98,6 +98,7 @@ class BatchOpts(C.Structure):
         ("block_size", C.c_int32), ("refill_quantum", C.c_int32), ("blocks_per_cu", C.c_int32),
         ("use_graph", C.c_int32), ("autotune", C.c_int32), ("scatter_quantum", C.c_int32),
         ("tally_copies", C.c_int32), ("entry_skip", C.c_int32), ("max_steps_per_photon", C.c_uint64),
+        ("record_fetch", C.c_int32), ("reserved0", C.c_int32),
     ]
 
 

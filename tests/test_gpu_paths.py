@@ -52,18 +52,40 @@ def test_tally_privatisation_depth_never_changes_results(ctx, small_meshes, copi
 
 @pytest.mark.parametrize("entry_skip", [1, 2])
 def test_entry_plane_skip_never_changes_results(ctx, small_meshes, entry_skip):
-    # Skipping the plane of the face a photon entered through removes a face that cannot be
-    # its exit: the same bits with the skip forced on or off, as with the default.
+    # Skipping the plane of the face a photon entered through (per-lane gathers) removes a
+    # face that cannot be its exit: the same bits with the skip forced on or off, as with the
+    # default.
     for name in ("slab", "ellipsoid"):
         m, mid, p, scans, seed = small_meshes[name]
         ref, _ = ctx.run_bscan(mid, p, scans, seed)
         o = O.BatchOpts()
-        o.entry_skip = entry_skip
+        o.record_fetch, o.entry_skip = 1, entry_skip
         got, _ = ctx.run_bscan(mid, p, scans, seed, opts=o)
         assert np.array_equal(ref, got), name
 
 
+@pytest.mark.parametrize("name", ["slab", "sphere", "ellipsoid"])
+def test_record_fetch_paths_give_the_same_bits(ctx, small_meshes, name):
+    # The cooperative fetch (four lanes read one record's planes into LDS, and a record stays
+    # staged while its photon stays in the tet) reads the same floats as the per-lane gathers:
+    # the same tallies, step counts and per-photon summaries, bit for bit, in plain and logged
+    # launches and at every block size the launch accepts.
+    m, mid, p, scans, seed = small_meshes[name]
+    runs = []
+    for fetch, skip, block in ((1, 2, 256), (1, 1, 256), (2, 0, 256), (2, 0, 128), (2, 0, 192), (0, 0, 0)):
+        o = O.BatchOpts()
+        o.record_fetch, o.entry_skip, o.block_size = fetch, skip, block
+        t, st = ctx.run_bscan(mid, p, scans, seed, opts=o)
+        tl, stl, rec = ctx.run_bscan(mid, p, scans, seed, opts=o, log=True)
+        assert np.array_equal(t, tl) and st.photon_steps == stl.photon_steps
+        runs.append((t, st.photon_steps, rec))
+    for t, steps, rec in runs[1:]:
+        assert np.array_equal(t, runs[0][0]) and steps == runs[0][1]
+        assert np.array_equal(rec, runs[0][2])
+
+
 @pytest.mark.parametrize("field,value", [("tally_copies", 17), ("entry_skip", 3), ("entry_skip", -1),
+                                         ("record_fetch", 3), ("record_fetch", -1),
                                          ("block_size", 96), ("block_size", 512)])
 def test_invalid_launch_options_are_rejected(ctx, small_meshes, field, value):
     m, mid, p, scans, seed = small_meshes["ellipsoid"]
@@ -280,7 +302,7 @@ def test_autotune_picks_a_valid_configuration(ctx, small_meshes):
     o, rate = ctx.autotune(mid, p, s, seed, calib=4000)
     assert o.block_size in (128, 192, 256) and o.refill_quantum in (12, 16, 20, 24) and rate > 0
     assert 16 <= o.scatter_quantum <= 44 and o.tally_copies in (1, 2, 4)
-    assert o.entry_skip in (0, 1, 2)
+    assert o.entry_skip in (0, 1, 2) and o.record_fetch in (0, 1)
     t1 = ctx.run_bscan(mid, p, s, seed)[0]
     t2 = ctx.run_bscan(mid, p, s, seed, opts=o)[0]
     assert np.array_equal(t1, t2)  # launch shape never changes results
