@@ -25,24 +25,26 @@ ALG_BYTES_PER_STEP = 84  # one tet record per photon-step: 4 planes x 16 B + 4 l
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E nominal
 LAYOUT_BYTES_PER_STEP = 80  # this build's packed tet record: 4 planes x 16 B + 4 packed neighbours x 4 B
 
-# Counter-derived figures of the batched slab4 launch (128 A-scans x 1e6 photons, quanta 24/20),
-# with the entry-plane skip on and off, each from the rocprofv3 summary named beside it; per
-# photon-step = counter / the launch's photon-steps.
+# Counter-derived figures of the batched slab4 launch (128 A-scans x 1e6 photons) for each record
+# path, each from the rocprofv3 summaries named beside it; per photon-step = counter / the
+# launch's photon-steps. HBM bytes = FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors by which
+# those counters read a copy kernel of known size).
 PROFILE = {
-    "skip_on": {"lane_loads_per_step": 4.16, "ta_busy": 0.77, "td_busy": 1.00, "valu_issue_upper": 1.10,
-                "valu_per_step": 5.75, "source": "profiles/transport_slab4_skip_counters.md"},
-    "skip_off": {"lane_loads_per_step": 5.01, "ta_busy": 0.85, "td_busy": 1.00, "valu_issue_upper": 1.04,
-                 "valu_per_step": 5.47, "source": "profiles/transport_slab4_memory.md, transport_slab4_sq.md"},
-    # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors by which those counters read a copy
-    # kernel of known size), per photon-step (skip off).
-    "hbm_bytes_per_step": 25.58, "hbm_source": "profiles/transport_slab4_pmc.md",
-    # The bump sample (64 of 1024 A-scans x 1e6 photons, quanta 40/16, skip off): HBM bytes
-    # per photon-step from its own FETCH_SIZE/WRITE_SIZE passes.
-    "bump_hbm_bytes_per_step": 47.35, "bump_hbm_source": "profiles/transport_bump_pmc.md",
+    # Cooperative record fetch, quanta 28/1.
+    "coop": {"l1_lookups_per_step": 1.72, "ta_busy": 0.72, "tcp_pending_l2": 0.45, "l2_hit": 0.72,
+             "l2_latency_cycles": 308, "wave_wait_dependency": 0.35, "wave_wait_issue": 0.28, "wave_issuing": 0.37,
+             "valu_per_step": 4.95, "hbm_bytes_per_step": 25.01,
+             "source": "profiles/transport_slab4_memory.md, transport_slab4_sq.md, transport_slab4_pmc.md"},
+    # Per-lane gathers, quanta 24/20, entry-plane skip on / off.
+    "lane_skip_on": {"l1_lookups_per_step": 4.16, "ta_busy": 0.77, "tcp_pending_l2": 0.15, "valu_per_step": 5.75,
+                     "hbm_bytes_per_step": 25.58, "source": "profiles/transport_slab4_perlane_skip_counters.md"},
+    "lane_skip_off": {"l1_lookups_per_step": 5.01, "ta_busy": 0.85, "tcp_pending_l2": 0.01, "valu_per_step": 5.47,
+                      "hbm_bytes_per_step": 25.58,
+                      "source": "profiles/transport_slab4_perlane_memory.md, transport_slab4_perlane_sq.md, "
+                                "transport_slab4_perlane_pmc.md"},
+    # The bump sample (64 of 1024 A-scans x 1e6 photons, cooperative fetch, quanta 32/16).
+    "bump_hbm_bytes_per_step": 44.05, "bump_hbm_source": "profiles/transport_bump_pmc.md",
 }
-# Records of a mesh up to this size stay in the Infinity Cache; the entry-plane skip defaults
-# to on for them (entry_skip 0 = auto).
-INFINITY_CACHE_BYTES = 256 << 20
 
 
 def log(msg):
@@ -65,12 +67,14 @@ def parse():
     ap.add_argument("--cpu-photons", type=int, default=3000, help="oracle photons per sampled A-scan")
     ap.add_argument("--cpu-scans", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true",
-                    help="skip the autotuner (context defaults: scatter/refill quanta 40/16, block 256)")
+                    help="skip the autotuner (context defaults: scatter/refill quanta 28/1, block 256)")
     ap.add_argument("--quanta", default="",
                     help="scatter,refill quanta to launch with instead of autotuning (e.g. 36,16: the same "
                          "launch a profiler run can then attribute to one kernel dispatch)")
     ap.add_argument("--entry-skip", type=int, default=-1, choices=(-1, 0, 1, 2),
-                    help="with --quanta: entry-plane skip of the launch (0 auto, 1 on, 2 off)")
+                    help="with --quanta: entry-plane skip of the per-lane record path (0 auto, 1 on, 2 off)")
+    ap.add_argument("--record-fetch", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="with --quanta: record path (0 auto, 1 per-lane gathers, 2 cooperative)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate of the C3 stream, req/s")
     ap.add_argument("--serve-window", type=float, default=0.1, help="seconds of arrivals in the C3 stream")
     ap.add_argument("--sweep-rates", default="10,25,50,100,200",
@@ -245,8 +249,10 @@ def main():
         opts.scatter_quantum, opts.refill_quantum = (int(v) for v in a.quanta.split(","))
         if a.entry_skip >= 0:
             opts.entry_skip = a.entry_skip
+        if a.record_fetch >= 0:
+            opts.record_fetch = a.record_fetch
         tuned = {"scatter_quantum": opts.scatter_quantum, "refill_quantum": opts.refill_quantum,
-                 "entry_skip": opts.entry_skip, "source": "--quanta"}
+                 "entry_skip": opts.entry_skip, "record_fetch": opts.record_fetch, "source": "--quanta"}
     elif not a.no_autotune:
         # The autotuner (coordinate search over launch options on a calibration slice of the
         # same B-scan: the first 1/div of every A-scan's photons), before the timed region;
@@ -255,13 +261,14 @@ def main():
         ts = time.perf_counter()
         o, rate = ctx.autotune(mid, params, scans, seed, calib=calib)
         opts.block_size, opts.refill_quantum, opts.scatter_quantum = o.block_size, o.refill_quantum, o.scatter_quantum
-        opts.tally_copies, opts.entry_skip = o.tally_copies, o.entry_skip
+        opts.tally_copies, opts.entry_skip, opts.record_fetch = o.tally_copies, o.entry_skip, o.record_fetch
         tuned = {"block_size": o.block_size, "refill_quantum": o.refill_quantum,
                  "scatter_quantum": o.scatter_quantum, "tally_copies": o.tally_copies, "entry_skip": o.entry_skip,
-                 "calibration_photons": calib, "seconds": time.perf_counter() - ts}
+                 "record_fetch": o.record_fetch, "calibration_photons": calib,
+                 "seconds": time.perf_counter() - ts}
         log(f"autotune ({tuned['seconds']:.1f} s): block {o.block_size} scatter/refill quanta "
             f"{o.scatter_quantum}/{o.refill_quantum} tally copies {o.tally_copies} entry skip {o.entry_skip} "
-            f"({rate:.3e} photons/s on the calibration slice)")
+            f"record fetch {o.record_fetch} ({rate:.3e} photons/s on the calibration slice)")
     # Warmup: the logged kernel variant once (a different kernel symbol, so a profiler's
     # per-kernel average of the unlogged single-set transport_kernel covers the timed launches
     # only), then W steps of the unlogged graph launch on a 1/div slice of the B-scan.
@@ -376,7 +383,7 @@ def main():
         bscans = bscans[::a.bump_stride]
         bmid = ctx.upload(bm)
         bo = O.BatchOpts()
-        bo.use_graph, bo.scatter_quantum, bo.refill_quantum, bo.entry_skip = 1, 40, 16, 2
+        bo.use_graph, bo.scatter_quantum, bo.refill_quantum, bo.record_fetch = 1, 32, 16, 2
         _, bst = ctx.run_bscan(bmid, bpar, bscans, bseed, opts=bo)
         ctx.release(bmid)
         del bm
@@ -384,7 +391,7 @@ def main():
         b_alg = bsteps * ALG_BYTES_PER_STEP / (bms * 1e-3) / 1e9
         b_hbm = bsteps * PROFILE["bump_hbm_bytes_per_step"] / (bms * 1e-3) / 1e9
         bump = {"workload": f"bump: 6x6x3 mm, 128x128x64 cells x 6 tets (6,291,456 tets), {len(bscans)} of 1024 "
-                            f"A-scans x 1e6 photons, 1024 bins to 2 mm, quanta 40/16, entry skip off",
+                            f"A-scans x 1e6 photons, 1024 bins to 2 mm, quanta 32/16, cooperative record fetch",
                 "photons": bst.photons, "kernel_ms": bms, "photons_per_s": bst.photons / (bms * 1e-3),
                 "photon_steps": bsteps, "steps_per_photon": bsteps / bst.photons,
                 "roofline": {"bound": "hbm", "achieved": b_alg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -408,6 +415,7 @@ def main():
         ts = time.perf_counter()
         t, st = ctx.run_bscan(mid, params, scans, seed, opts=opts)
         walls.append(time.perf_counter() - ts)
+        path = "coop" if st.record_fetch == 2 else ("lane_skip_on" if st.entry_skip else "lane_skip_off")
         kms.append(st.kernel_ms)
         steps.append(st.photon_steps)
         iters.append(st.wave_iterations)
@@ -420,20 +428,22 @@ def main():
     kernel_s = sum(kms) * 1e-3
     steps_per_launch = sum(steps) / n_steps
     launch_s = kernel_s / n_steps
+    prof = PROFILE[path]
     achieved = steps_per_launch * ALG_BYTES_PER_STEP / launch_s / 1e9
-    hbm_measured = steps_per_launch * PROFILE["hbm_bytes_per_step"] / launch_s / 1e9
+    hbm_measured = steps_per_launch * prof["hbm_bytes_per_step"] / launch_s / 1e9
     lane_util = sum(steps) / (64.0 * max(1, sum(iters)))
     if served_same is not None:
         served_same["equal_to_batched"] = bool(np.array_equal(served_out, t))
 
-    skip_on = opts.entry_skip == 1 or (opts.entry_skip == 0 and mesh.num_tets * 128 <= INFINITY_CACHE_BYTES)
-    prof = PROFILE["skip_on" if skip_on else "skip_off"]
     gather = None
     if gather_peak:
-        got = sum(steps) / kernel_s * prof["lane_loads_per_step"]
-        gather = {"bound": "vector-memory gathers", "achieved": got, "peak": gather_peak,
-                  "unit": "16-byte lane-loads/s", "frac": got / gather_peak,
-                  "lane_loads_per_step": prof["lane_loads_per_step"], "entry_skip": skip_on,
+        # The L1 looks up one line per lane of a divergent vector load per cycle (four lanes
+        # reading one 64-byte piece share a lookup): the probe's rate of divergent 16-byte
+        # loads is the ceiling of L1 lookups.
+        got = sum(steps) / kernel_s * prof["l1_lookups_per_step"]
+        gather = {"bound": "L1 lookups", "achieved": got, "peak": gather_peak,
+                  "unit": "L1 lookups/s", "frac": got / gather_peak,
+                  "l1_lookups_per_step": prof["l1_lookups_per_step"], "record_path": path,
                   "peak_source": "octmps_probe_gather: 5 x 16 B from pseudo-random 128-byte records of a "
                                  "2 MiB set, 6 waves per SIMD"}
     ctx.close()
@@ -464,7 +474,7 @@ def main():
         "batched_over_plain": (value / plain["photons_per_s"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": frac,
-                     "traffic": steps_per_launch * PROFILE["hbm_bytes_per_step"],
+                     "traffic": steps_per_launch * prof["hbm_bytes_per_step"],
                      "per_unit": f"{ALG_BYTES_PER_STEP} B per photon-step",
                      "binding": frac <= 1.0,
                      "note": ("algorithmic bytes (one tet record per photon-step) over the nominal HBM peak; "
@@ -473,13 +483,15 @@ def main():
                               "algorithmic bytes (one tet record per photon-step) over the nominal HBM peak"),
                      "peak_kind": "nominal MI355X HBM3E", "measured_copy_GB_s": hbm_copy,
                      "layout_bytes_per_step": LAYOUT_BYTES_PER_STEP,
-                     "measured_hbm": {"bytes_per_step": PROFILE["hbm_bytes_per_step"], "GB_s": hbm_measured,
-                                      "frac": hbm_measured / HBM_PEAK_GBS, "source": PROFILE["hbm_source"]}},
-        "bound_by": {"resource": "vector-memory gather path (TD busy for the whole launch) together with VALU issue",
-                     "frac": prof["td_busy"], "gather_frac": gather["frac"] if gather else None,
-                     "ta_busy": prof["ta_busy"], "valu_issue_upper_bound": prof["valu_issue_upper"],
-                     "valu_per_step": prof["valu_per_step"], "source": prof["source"],
-                     "entry_skip": skip_on, "lane_utilisation": lane_util},
+                     "measured_hbm": {"bytes_per_step": prof["hbm_bytes_per_step"], "GB_s": hbm_measured,
+                                      "frac": hbm_measured / HBM_PEAK_GBS, "source": prof["source"]}},
+        "bound_by": {"resource": ("latency of L1 misses: the L1 waits on L2 data for much of the launch, each "
+                                  "wave waits on a dependency a third of its time, at 6 waves per SIMD"
+                                  if path == "coop" else
+                                  "L1 lookups of the per-lane record gathers (one per lane per 16-byte load)"),
+                     "record_path": path, "gather_frac": gather["frac"] if gather else None,
+                     "counters": {k: v for k, v in prof.items() if k != "source"}, "source": prof["source"],
+                     "lane_utilisation": lane_util},
         "gather_roofline": gather,
         "bump_sample": bump,
         "cpu_baseline": cpu,

@@ -254,6 +254,8 @@ typedef struct {
                                  photon_steps / (64 * wave_iterations) */
     uint64_t bad_contributions; /* detector contributions the tally could not hold exactly
                                    (expected 0; see the tally layout above) */
+    uint32_t record_fetch;    /* record path the launch ran: 1 per-lane gathers, 2 cooperative */
+    uint32_t entry_skip;      /* 1 when the per-lane gathers skipped the entry plane */
 } octmps_run_stats;
 
 typedef struct {
@@ -319,7 +321,7 @@ octmps_status octmps_run_bscan_logged(octmps_ctx *ctx, int32_t mesh_id, const oc
                                       uint64_t log_records);
 /* Autotune scatter quantum, block size, refill quantum, tally privatisation depth and the
  * entry-plane skip on a calibration slice of a B-scan (coordinate search from the context
- * defaults, 18 launches: a candidate must be more than 1 % faster to win, and the winner must
+ * defaults, 19 launches: a candidate must be more than 1 % faster to win, and the winner must
  * lead the starting point again on a second measurement of both);
  * writes the winner into *opts (scatter_quantum, block_size, refill_quantum, tally_copies,
  * entry_skip, record_fetch), makes the first four the context's default (entry_skip and

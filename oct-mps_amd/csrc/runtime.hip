@@ -143,10 +143,12 @@ octmps_status make_scan(octmps_ctx *ctx, const MeshEntry &me, const octmps_param
 }
 
 int default_block(octmps_ctx *ctx) { return ctx->tuned_block > 0 ? ctx->tuned_block : 256; }
-// Untuned defaults: measured best on the four-layer slab B-scan (128 A-scans x 1e5 photons:
-// scatter/refill quanta 24/20 in 2334 ms, 32/16 in 2357 ms, 40/16 in 2446 ms).
-int default_quantum(octmps_ctx *ctx) { return ctx->tuned_q > 0 ? ctx->tuned_q : 20; }
-int default_scatter(octmps_ctx *ctx) { return ctx->tuned_sq > 0 ? ctx->tuned_sq : 24; }
+// Untuned defaults (cooperative record fetch). The best refill quantum depends on the mesh:
+// on the four-layer slab B-scan (128 A-scans x 1e5 photons) scatter/refill quanta 28/1 take
+// 1901 ms, 28/4 1941 ms, 24/16 2097 ms; on the bump mesh (every 64th of its 1024 A-scans x
+// 1e6 photons) 32/16 take 4700 ms and 32/2 4852 ms. The autotuner searches both ranges.
+int default_quantum(octmps_ctx *ctx) { return ctx->tuned_q > 0 ? ctx->tuned_q : 1; }
+int default_scatter(octmps_ctx *ctx) { return ctx->tuned_sq > 0 ? ctx->tuned_sq : 28; }
 int default_copies(octmps_ctx *ctx) { return ctx->tuned_copies > 0 ? ctx->tuned_copies : 1; }
 
 // The cooperative record fetch by default: when its LDS lets a CU hold as many waves as the
@@ -376,6 +378,8 @@ octmps_status run_scans(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p
         stats->refill_quantum = (uint32_t)q;
         stats->wave_iterations = ctrl[CTRL_ITERS];
         stats->bad_contributions = ctrl[CTRL_BAD];
+        stats->record_fetch = coop ? 2u : 1u;
+        stats->entry_skip = (!coop && A.skip_entry) ? 1u : 0u;
     }
     return OCTMPS_OK;
 }
@@ -475,7 +479,7 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     // (per-lane gathers with or without the entry-plane skip). Launch options move a calibration launch by a few percent at most, so a candidate
     // replaces the incumbent only when it is more than 1 % faster, and the final winner must
     // beat the starting point again on a second measurement of both: the noise of a single
-    // launch no longer picks the options (18 calibration launches).
+    // launch no longer picks the options (19 calibration launches).
     constexpr double kMargin = 1.01;
     octmps_batch_opts start = *opts;
     start.block_size = default_block(c);
@@ -504,12 +508,12 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
         }
         return st;
     };
-    for (int sq : {20, 28, 32, 40}) {
+    for (int sq : {20, 24, 32, 40}) {
         octmps_batch_opts o = win;
         o.scatter_quantum = sq;
         if (sq != win.scatter_quantum && (s = trial(o)) != OCTMPS_OK) return s;
     }
-    for (int q : {12, 16, 24}) {
+    for (int q : {2, 4, 8, 16}) {
         octmps_batch_opts o = win;
         o.refill_quantum = q;
         if (q != win.refill_quantum && (s = trial(o)) != OCTMPS_OK) return s;

@@ -87,6 +87,7 @@ class RunStats(C.Structure):
         ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
         ("launches", C.c_uint32), ("block_size", C.c_uint32), ("grid_size", C.c_uint32),
         ("refill_quantum", C.c_uint32), ("wave_iterations", C.c_uint64), ("bad_contributions", C.c_uint64),
+        ("record_fetch", C.c_uint32), ("entry_skip", C.c_uint32),
     ]
 
     def as_dict(self):

@@ -300,7 +300,7 @@ def test_gather_probe_measures_a_sane_ceiling(ctx):
 def test_autotune_picks_a_valid_configuration(ctx, small_meshes):
     _, mid, p, s, seed = small_meshes["slab"]
     o, rate = ctx.autotune(mid, p, s, seed, calib=4000)
-    assert o.block_size in (128, 192, 256) and o.refill_quantum in (12, 16, 20, 24) and rate > 0
+    assert o.block_size in (128, 192, 256) and o.refill_quantum in (1, 2, 4, 8, 16) and rate > 0
     assert 16 <= o.scatter_quantum <= 44 and o.tally_copies in (1, 2, 4)
     assert o.entry_skip in (0, 1, 2) and o.record_fetch in (0, 1)
     t1 = ctx.run_bscan(mid, p, s, seed)[0]
