@@ -366,9 +366,12 @@ def main():
     # Ceiling of the vector-memory path for divergent record gathers (cache-resident record
     # set), and the measured HBM copy rate beside the nominal peak the roofline uses: 4 GiB
     # read + 4 GiB written per copy (16x the Infinity Cache).
-    gather_peak = hbm_copy = None
+    gather_peak = hbm_copy = l1_probe = None
     if not a.no_probes:
         gather_peak = ctx.probe_gather()
+        l1_probe = {name: ctx.probe_l1(i)[1] for i, name in enumerate(
+            ("divergent", "quad_coalesced", "half_masked", "half_out_of_range", "quad_coalesced_lds_dma"))}
+        log("L1 probe, CU cycles per 16-byte wave load: " + ", ".join(f"{k} {v:.1f}" for k, v in l1_probe.items()))
         nb = 4 << 30
         ms = ctx.probe_copy(nb, 5)
         hbm_copy = 2 * nb / (ms * 1e-3) / 1e9
@@ -445,7 +448,8 @@ def main():
                   "unit": "L1 lookups/s", "frac": got / gather_peak,
                   "l1_lookups_per_step": prof["l1_lookups_per_step"], "record_path": path,
                   "peak_source": "octmps_probe_gather: 5 x 16 B from pseudo-random 128-byte records of a "
-                                 "2 MiB set, 6 waves per SIMD"}
+                                 "2 MiB set, 6 waves per SIMD",
+                  "l1_probe_cu_cycles_per_wave_load": l1_probe}
     ctx.close()
     frac = achieved / HBM_PEAK_GBS
     line = {

@@ -428,6 +428,16 @@ octmps_status octmps_probe_copy(octmps_ctx *ctx, uint64_t bytes, int32_t repeats
  * lane-loads per second. n_records in 1..2^24. */
 octmps_status octmps_probe_gather(octmps_ctx *ctx, int32_t n_records, int32_t iters, double *lane_loads_per_s);
 
+/* L1 cost of a 16-byte vector load by access pattern (no counterpart in the reference): every
+ * lane of a 6-waves-per-SIMD grid issues iters x 8 loads from a 16 KiB (L1-resident) buffer.
+ * pattern 0: divergent (the 64 lanes of a load hit 64 cache lines); 1: quad-coalesced (each
+ * four lanes read the four quarters of one 64-byte piece); 2: divergent, every other lane
+ * masked off; 3: divergent, every other lane out of range of the buffer resource; 4: pattern
+ * 1 as DMA loads into LDS (the cooperative record fetch). Returns wave-level load
+ * instructions per second over the GPU and CU cycles per load at the nominal clock. */
+octmps_status octmps_probe_l1(octmps_ctx *ctx, int32_t pattern, int32_t iters, double *loads_per_s,
+                              double *cu_cycles_per_load);
+
 #ifdef __cplusplus
 }
 #endif

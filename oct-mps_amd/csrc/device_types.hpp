@@ -199,5 +199,7 @@ hipError_t launch_probe_tally(const KArgs &a, const DevSet &S, int n, const floa
 // blocks x 256 lanes each read iters records (5 x 16 B) of recs[0..n).
 hipError_t launch_gather_probe(const TetRec *recs, uint32_t n, int iters, uint32_t *out, int blocks,
                                hipStream_t stream);
+// blocks x 256 lanes each issue iters x 8 loads of `pattern` from a 16 KiB buffer (buf).
+hipError_t launch_l1_probe(int pattern, const uint8_t *buf, int iters, float *out, int blocks, hipStream_t stream);
 
 }  // namespace octmps

@@ -690,6 +690,39 @@ extern "C" octmps_status octmps_probe_gather(octmps_ctx *c, int32_t n_records, i
     return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_gather");
 }
 
+extern "C" octmps_status octmps_probe_l1(octmps_ctx *c, int32_t pattern, int32_t iters, double *loads_per_s,
+                                        double *cu_cycles_per_load) {
+    if (!c || pattern < 0 || pattern > 4 || iters <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
+    std::lock_guard<std::mutex> lk(c->run_mu);
+    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
+    const int blocks = c->n_cu * 6;  // 6 waves per SIMD, as the transport kernel
+    int khz = 0;
+    hipError_t e = hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, c->device);
+    if (e != hipSuccess) return hip_fail(e, "clock rate");
+    void *buf = nullptr, *out = nullptr;
+    if ((e = hipMalloc(&buf, 16384)) != hipSuccess) return hip_fail(e, "probe allocation");
+    if ((e = hipMalloc(&out, (size_t)blocks * 256 * sizeof(float))) != hipSuccess) {
+        (void)hipFree(buf);
+        return hip_fail(e, "probe allocation");
+    }
+    e = hipMemsetAsync(buf, 0, 16384, c->stream);
+    if (e == hipSuccess) e = launch_l1_probe(pattern, (const uint8_t *)buf, 16, (float *)out, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess) e = launch_l1_probe(pattern, (const uint8_t *)buf, iters, (float *)out, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    float ms = 0;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    (void)hipFree(out);
+    const double loads = (double)blocks * 4.0 * iters * 8.0;  // wave-level load instructions
+    const double rate = ms > 0 ? loads / (ms * 1e-3) : 0.0;
+    if (loads_per_s) *loads_per_s = rate;
+    if (cu_cycles_per_load) *cu_cycles_per_load = rate > 0 ? c->n_cu * (khz * 1e3) / rate : 0.0;
+    return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_l1");
+}
+
 extern "C" octmps_status octmps_probe_copy(octmps_ctx *c, uint64_t bytes, int32_t repeats, double *ms) {
     if (!c || bytes < 16 || repeats <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
     std::lock_guard<std::mutex> lk(c->run_mu);

@@ -1079,7 +1079,56 @@ __global__ void __launch_bounds__(256) gather_probe_kernel(const TetRec *recs, u
     }
     out[g] = acc;
 }
+
+// L1 cost of one 16-byte vector load per access pattern: blocks x 256 lanes issue iters x 8
+// loads each from a 16 KiB buffer (128 lines of 128 B, L1-resident). Patterns: 0 divergent
+// (the 64 lanes of a load hit 64 lines), 1 quad-coalesced (lanes 4q..4q+3 read the four
+// 16-byte quarters of one 64-byte piece, 16 lines per load), 2 divergent with every other
+// lane masked off, 3 divergent with every other lane out of range, 4 quad-coalesced DMA into
+// LDS (buffer_load ... lds, the cooperative record fetch's form). Every offset is below
+// 16 KiB except pattern 3's out-of-range ones (past the buffer resource's size).
+template <int P>
+__global__ void __launch_bounds__(256) l1_probe_kernel(const uint8_t *buf, int iters, float *out) {
+    __shared__ float4 dma_dst[4 * 128];
+    const uint32_t l = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)buf, (short)0, 16384, kBufferWord3);
+    float acc = 0.0f;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            uint32_t off;
+            if (P == 1 || P == 4) off = (((l >> 2) * 9u + u * 5u + (uint32_t)it) & 127u) * 128u + ((u >> 2) & 1u) * 64u + (l & 3u) * 16u;
+            else off = (((l * 5u + u * 17u + (uint32_t)it * 3u) & 127u) * 128u) + (u & 3u) * 16u;
+            if (P == 3 && (l & 1u)) off = kOutOfRange;
+            if (P == 4) {
+                typedef __attribute__((address_space(3))) void *lds_ptr;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_ptr)(dma_dst + wave * 128u + (u & 1u) * 64u), 16, off, 0, 0, 0);
+            } else if (P != 2 || (l & 1u) == 0) {
+                const u4v v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+                acc += __uint_as_float(v.x) + __uint_as_float(v.y) + __uint_as_float(v.z) + __uint_as_float(v.w);
+            }
+        }
+    }
+    if (P == 4) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        asm volatile("" ::: "memory");
+        acc += dma_dst[wave * 128u + l].x;
+    }
+    out[blockIdx.x * 256u + threadIdx.x] = acc;
+}
 }  // namespace
+
+hipError_t launch_l1_probe(int pattern, const uint8_t *buf, int iters, float *out, int blocks, hipStream_t stream) {
+    switch (pattern) {
+    case 0: hipLaunchKernelGGL(l1_probe_kernel<0>, dim3(blocks), dim3(256), 0, stream, buf, iters, out); break;
+    case 1: hipLaunchKernelGGL(l1_probe_kernel<1>, dim3(blocks), dim3(256), 0, stream, buf, iters, out); break;
+    case 2: hipLaunchKernelGGL(l1_probe_kernel<2>, dim3(blocks), dim3(256), 0, stream, buf, iters, out); break;
+    case 3: hipLaunchKernelGGL(l1_probe_kernel<3>, dim3(blocks), dim3(256), 0, stream, buf, iters, out); break;
+    case 4: hipLaunchKernelGGL(l1_probe_kernel<4>, dim3(blocks), dim3(256), 0, stream, buf, iters, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_gather_probe(const TetRec *recs, uint32_t n, int iters, uint32_t *out, int blocks,
                                hipStream_t stream) {

@@ -193,6 +193,7 @@ SIGNATURES = [
                                      C.POINTER(C.c_float), _u64p, _u64p]),
     ("octmps_probe_copy", C.c_int, [_P, C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
     ("octmps_probe_gather", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    ("octmps_probe_l1", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 
 
@@ -512,6 +513,14 @@ class Context:
         r = C.c_double(0)
         check(lib.octmps_probe_gather(self.h, n_records, iters, C.byref(r)), "probe_gather")
         return r.value
+
+    def probe_l1(self, pattern, iters=2048):
+        """(wave load instructions/s, CU cycles per load at the nominal clock) of an L1 access
+        pattern: 0 divergent, 1 quad-coalesced, 2 half the lanes masked, 3 half out of range,
+        4 quad-coalesced DMA into LDS."""
+        r, cyc = C.c_double(0), C.c_double(0)
+        check(lib.octmps_probe_l1(self.h, pattern, iters, C.byref(r), C.byref(cyc)), "probe_l1")
+        return r.value, cyc.value
 
     def probe_sampler(self, kind, param, u):
         u = np.ascontiguousarray(u, np.float32)

@@ -297,6 +297,19 @@ def test_gather_probe_measures_a_sane_ceiling(ctx):
         ctx.probe_gather(0, 10)
 
 
+def test_l1_probe_shows_the_lookup_cost_of_each_pattern(ctx):
+    # The cost model behind the cooperative record fetch: the L1 looks up one line per cycle
+    # and serves four lanes of a 16-byte load per cycle, so a divergent wave load takes ~64
+    # CU cycles, a quad-coalesced one ~16 (also as DMA into LDS), and masked-off or
+    # out-of-range lanes cost nothing.
+    cyc = [ctx.probe_l1(p)[1] for p in range(5)]
+    assert 40 < cyc[0] < 90
+    assert cyc[1] < cyc[0] / 3 and cyc[4] < cyc[0] / 3
+    assert cyc[2] < 0.7 * cyc[0] and cyc[3] < 0.7 * cyc[0]
+    with pytest.raises(O.OctmpsError):
+        ctx.probe_l1(5)
+
+
 def test_autotune_picks_a_valid_configuration(ctx, small_meshes):
     _, mid, p, s, seed = small_meshes["slab"]
     o, rate = ctx.autotune(mid, p, s, seed, calib=4000)
