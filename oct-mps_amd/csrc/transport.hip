@@ -101,46 +101,47 @@ __device__ __forceinline__ float fresnel_f(float ni, float nt, float ca1) {
     return r;
 }
 
-// SpinTheta, src/octmps_kernel.cuh:807-820 (hg_a = 1-g^2, hg_b = 1/(2g)).
+// The samplers and the rotation below compute every case and select the result, so a wave
+// whose lanes take different cases runs no divergent branch (the loop is bound by
+// instruction issue, and a divergent branch costs several scalar instructions).
+
+// SpinTheta, src/octmps_kernel.cuh:807-820 (hg_a = 1-g^2, hg_b = 1/(2g); g == 0 gives
+// hg_a = 1, hg_b = 0, so the discarded general case stays finite).
 __device__ __forceinline__ float hg_cos(float g, float hg_a, float hg_b, float u_oc) {
-    if (g == 0.0f) return 2.0f * u_oc - 1.0f;
-    float t = hg_a * rcp(fmaf(2.0f * g, u_oc, 1.0f - g));
-    float c = fmaf(-t, t, fmaf(g, g, 1.0f)) * hg_b;
-    return fminf(fmaxf(c, -1.0f), 1.0f);
+    const float t = hg_a * rcp(fmaf(2.0f * g, u_oc, 1.0f - g));
+    const float c = fminf(fmaxf(fmaf(-t, t, fmaf(g, g, 1.0f)) * hg_b, -1.0f), 1.0f);
+    return g == 0.0f ? 2.0f * u_oc - 1.0f : c;
 }
 
 // SpinThetaForwardFstBias, src/octmps_kernel.cuh:881-896.
 __device__ __forceinline__ float bias_cos(float a, float inv_1ma, float inv_sq, float inv_2a,
                                           float opa2, float u_oc) {
-    if (a == 0.0f) return u_oc;
-    float t = fmaf(u_oc, inv_1ma, (1.0f - u_oc) * inv_sq);
-    float c = (opa2 - rcp(t * t)) * inv_2a;
-    return fminf(fmaxf(c, -1.0f), 1.0f);
+    const float t = fmaf(u_oc, inv_1ma, (1.0f - u_oc) * inv_sq);
+    const float c = fminf(fmaxf((opa2 - rcp(t * t)) * inv_2a, -1.0f), 1.0f);
+    return a == 0.0f ? u_oc : c;
 }
 
 // Rotation of Spin / SpinBias (src/octmps_kernel.cuh:850-866, 1009-1018) around axis
 // (ax, ay, az), written with sqrt(ax^2+ay^2) in place of sqrt(1-az^2) and renormalised
 // like the reference's SINGLE_PRECISION variant (src/octmps_kernel.cuh:870-877).
 // copysign_mode: 0 -> uz = cost*sign(az) (Spin), 1 -> copysign(cost, az*cost) (SpinBias).
+// The reciprocal square roots are v_rsq_f32 on normal arguments (sp2 > 1e-12 where the
+// general case is kept, a squared norm near 1), where it equals rsqrtf bit for bit.
 __device__ __forceinline__ void rotate(float ax, float ay, float az, float cost, float r_psi,
                                        int copysign_mode, float &ux, float &uy, float &uz) {
-    float sint = fsqrt(fmaxf(fmaf(-cost, cost, 1.0f), 0.0f));
+    const float sint = fsqrt(fmaxf(fmaf(-cost, cost, 1.0f), 0.0f));
     // psi = 2 pi r: v_sin_f32 / v_cos_f32 take their argument in revolutions.
     const float sinp = __builtin_amdgcn_sinf(r_psi), cosp = __builtin_amdgcn_cosf(r_psi);
-    float stcp = sint * cosp, stsp = sint * sinp;
-    float sp2 = fmaf(ax, ax, ay * ay);
-    float nx, ny, nz;
-    if (sp2 <= kNormalSin2) {
-        nx = stcp;
-        ny = stsp;
-        nz = copysign_mode ? copysignf(cost, az * cost) : cost * (az >= 0.0f ? 1.0f : -1.0f);
-    } else {
-        float rs = rsqrtf(sp2);
-        nx = fmaf(fmaf(stcp * ax, az, -stsp * ay), rs, ax * cost);
-        ny = fmaf(fmaf(stcp * ay, az, stsp * ax), rs, ay * cost);
-        nz = fmaf(-stcp, sp2 * rs, az * cost);
-    }
-    float rn = rsqrtf(fmaf(nx, nx, fmaf(ny, ny, nz * nz)));
+    const float stcp = sint * cosp, stsp = sint * sinp;
+    const float sp2 = fmaf(ax, ax, ay * ay);
+    const bool normal = sp2 <= kNormalSin2;
+    const float rs = __builtin_amdgcn_rsqf(sp2);
+    const float gx = fmaf(fmaf(stcp * ax, az, -stsp * ay), rs, ax * cost);
+    const float gy = fmaf(fmaf(stcp * ay, az, stsp * ax), rs, ay * cost);
+    const float gz = fmaf(-stcp, sp2 * rs, az * cost);
+    const float zn = copysign_mode ? copysignf(cost, az * cost) : cost * (az >= 0.0f ? 1.0f : -1.0f);
+    const float nx = normal ? stcp : gx, ny = normal ? stsp : gy, nz = normal ? zn : gz;
+    const float rn = __builtin_amdgcn_rsqf(fmaf(nx, nx, fmaf(ny, ny, nz * nz)));
     ux = nx * rn;
     uy = ny * rn;
     uz = nz * rn;
@@ -229,9 +230,11 @@ __device__ void tally_detection(const KArgs &A, const DevSet &S, uint32_t slot, 
     }
 }
 
-__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
-    uint32_t lo = __shfl((unsigned int)(v & 0xffffffffull), src);
-    uint32_t hi = __shfl((unsigned int)(v >> 32), src);
+// Value of lane `src` (wave-uniform) as a scalar: the refill decisions made from it stay
+// uniform branches, and the loop's exit stays a uniform branch.
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
     return ((uint64_t)hi << 32) | lo;
 }
 
@@ -380,21 +383,63 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
         const int r2 = __builtin_amdgcn_mov_dpp(own, 0xAA, 0xF, 0xF, false);
         const int r3 = __builtin_amdgcn_mov_dpp(own, 0xFF, 0xF, 0xF, false);
         const uint32_t quarter = (lane & 3u) << 4;
-        // The wave's stage (wave-uniform, so the LDS base can go in M0).
-        float4 *stage = lds_stage + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * kStageF4;
+        // The five loads run under their own lane masks, set in one asm block: written as five
+        // masked builtins, each mask costs a save, a branch and a restore of the exec mask
+        // (scalar instructions, which the loop is short of). M0 holds the LDS destination of
+        // each load (the wave's stage, wave-uniform); the LDS writes complete under vmcnt,
+        // which the crossing phase waits for before it reads the stage. The exec mask is
+        // restored before the block ends. The buffer resource is the one of tets_rsrc: base
+        // address, stride 0, tets_bytes records (out-of-range offsets read nothing), raw dwords.
         typedef __attribute__((address_space(3))) void *lds_ptr;
-        if (r0 != -1)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage), 16, (uint32_t)r0 + quarter, 0, 0, 0);
-        if (r1 != -1)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage + 65), 16, (uint32_t)r1 + quarter, 0, 0, 0);
-        if (r2 != -1)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage + 130), 16, (uint32_t)r2 + quarter, 0, 0, 0);
-        if (r3 != -1)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage + 195), 16, (uint32_t)r3 + quarter, 0, 0, 0);
-        if (want) {
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(tets_rsrc, (lds_ptr)(stage + kStageLinks), 16, (uint32_t)own + 64u, 0, 0, 0);
-            flags |= F_STAGED;
-        }
+        const uint32_t sb = (uint32_t)(uintptr_t)(lds_ptr)lds_stage +
+                            __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * (kStageF4 * 16u);
+        const uint64_t k0 = __ballot(r0 != -1), k1 = __ballot(r1 != -1), k2 = __ballot(r2 != -1),
+                       k3 = __ballot(r3 != -1), k4 = __ballot(want);
+        const uint64_t rbase = (uint64_t)(uintptr_t)A.tets;
+        u4v rs;
+        rs.x = (uint32_t)rbase;
+        rs.y = (uint32_t)(rbase >> 32) & 0xFFFFu;
+        rs.z = A.tets_bytes;
+        rs.w = (uint32_t)kBufferWord3;
+        const uint32_t a0 = (uint32_t)r0 + quarter, a1 = (uint32_t)r1 + quarter, a2 = (uint32_t)r2 + quarter,
+                       a3 = (uint32_t)r3 + quarter, a4 = (uint32_t)own + 64u;
+        static_assert(65 * 16 == 0x410 && 130 * 16 == 0x820 && 195 * 16 == 0xc30 && kStageLinks * 16 == 0x1040,
+                      "stage offsets of the fetch block");
+        uint64_t saved;
+        // M0 is a reserved register: the compiler also sets it for LDS DMA loads it emits
+        // itself, and this kernel variant emits none (a CPU test compiles the kernel and
+        // checks that M0 is written only by these blocks).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+        asm volatile(
+            "s_mov_b64 %[sv], exec\n\t"
+            "s_mov_b32 m0, %[sb]\n\t"
+            "s_mov_b64 exec, %[k0]\n\t"
+            "s_nop 0\n\t"
+            "buffer_load_dwordx4 %[a0], %[rs], 0 offen lds\n\t"
+            "s_add_u32 m0, %[sb], 0x410\n\t"
+            "s_mov_b64 exec, %[k1]\n\t"
+            "s_nop 0\n\t"
+            "buffer_load_dwordx4 %[a1], %[rs], 0 offen lds\n\t"
+            "s_add_u32 m0, %[sb], 0x820\n\t"
+            "s_mov_b64 exec, %[k2]\n\t"
+            "s_nop 0\n\t"
+            "buffer_load_dwordx4 %[a2], %[rs], 0 offen lds\n\t"
+            "s_add_u32 m0, %[sb], 0xc30\n\t"
+            "s_mov_b64 exec, %[k3]\n\t"
+            "s_nop 0\n\t"
+            "buffer_load_dwordx4 %[a3], %[rs], 0 offen lds\n\t"
+            "s_add_u32 m0, %[sb], 0x1040\n\t"
+            "s_mov_b64 exec, %[k4]\n\t"
+            "s_nop 0\n\t"
+            "buffer_load_dwordx4 %[a4], %[rs], 0 offen lds\n\t"
+            "s_mov_b64 exec, %[sv]"
+            : [sv] "=&s"(saved)
+            : [k0] "s"(k0), [k1] "s"(k1), [k2] "s"(k2), [k3] "s"(k3), [k4] "s"(k4), [a0] "v"(a0), [a1] "v"(a1),
+              [a2] "v"(a2), [a3] "v"(a3), [a4] "v"(a4), [sb] "s"(sb), [rs] "s"(rs)
+            : "memory", "m0", "scc");
+#pragma clang diagnostic pop
+        if (want) flags |= F_STAGED;
     };
 
     for (;;) {
@@ -408,7 +453,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                 uint64_t base = 0;
                 if ((int)lane == leader)
                     base = atomicAdd((unsigned long long *)&A.ctrl[CTRL_NEXT_ID], (unsigned long long)ndead);
-                base = shfl_u64(base, leader);
+                base = readlane_u64(base, leader);
                 if (!alive) {
                     const uint32_t rank = (uint32_t)__popcll(dmask & ((1ull << lane) - 1ull));
                     const uint64_t nid = base + rank;
@@ -525,12 +570,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             coop_fetch(ready && !(steps + 1u > max_steps) && !(flags & F_STAGED));
         }
 
-        if (!ready) {
-            // parked or dead lane: nothing to do in the crossing phase
-        } else if (++steps > max_steps) {
-            finish = true;
-            killed = true;
-        } else {
+        // Step guard: a photon past max_steps is stopped and counted.
+        steps = ready ? steps + 1u : steps;
+        killed = ready && steps > max_steps;
+        finish = killed;
+        // Lanes whose step ended in this phase (a face into the same region), for roulette.
+        bool crossed = false;
+        // One divergent branch: every outcome inside is a select, since a divergent branch
+        // costs several scalar instructions and the loop is bound by instruction issue.
+        if (ready && !killed) {
             // ---- ComputeStepSize, src/octmps_kernel.cuh:505-515. A fresh step length
             // (sleft == 0) is normally drawn ahead, where the previous step ended (parked
             // phase, launch), so this phase rarely runs the logarithm. +0 marks "draw now";
@@ -538,7 +586,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             float tau_d = tau;
             if (__float_as_uint(tau_d) == 0u) tau_d = neg_ln(rand_oc(sG, aG));
             float s = tau_d * rmuas;
-            tau = 0.0f;
             // ---- HitBoundary, src/octmps_kernel.cuh:517-572
             // The tet record is read through a buffer resource (32-bit offsets). With
             // SKIP, a photon that crossed into this tet through face k does not fetch
@@ -598,57 +645,62 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             face(2, p2, c2);
             face(3, p3, c3);
             const bool hit = imin >= 0 && s > dmin;
-            if (hit) {
-                // Glass (mus = 0) keeps the leftover unchanged, as (s - d) / DBL_MAX does
-                // in the reference; an overflowed step stays infinite, and a zero leftover
-                // is stored as +0 so that the next step draws.
-                tau = (muas == 0.0f) ? (isinf(s) ? INFINITY : tau_d + 0.0f) : (s - dmin) * muas;
-                s = dmin;
-            }
+            // The leftover of a step that reaches a face, in optical depth. Glass (mus = 0)
+            // keeps the leftover unchanged, as (s - d) / DBL_MAX does in the reference; an
+            // overflowed step stays infinite, and a zero leftover is stored as +0 so that the
+            // next step draws.
+            const float left_m = (s - dmin) * muas;
+            const float left_g = __builtin_isinf(s) ? INFINITY : tau_d + 0.0f;
+            tau = hit ? (muas == 0.0f ? left_g : left_m) : 0.0f;
+            s = hit ? dmin : s;
             // ---- Hop, src/octmps_kernel.cuh:574-582
             x = fmaf(s, ux, x);
             y = fmaf(s, uy, y);
             z = fmaf(s, uz, z);
             op += s;
-            if (maxz < z) maxz = z;
-
-            if (hit) {
-                // Packed neighbour word: pool tet id (all ones at the mesh boundary), the
-                // shared face's index within the neighbour, and the neighbour's region.
-                const uint32_t pk = imin == 0 ? L.x : (imin == 1 ? L.y : (imin == 2 ? L.z : L.w));
-                // Same region: the region field equals the photon's and is not the escape
-                // (which the mesh boundary's all-ones word also carries).
-                const uint32_t field = pk >> kRegionShift;
-                if (field == (uint32_t)region && field != kRegionEscape && -mincos >= kCos90F) {
-                    // FastReflectTransmit (src/octmps_kernel.cuh:590-805) on a face into the
-                    // same region away from grazing incidence: equal indices give R = 0
-                    // (sam == 0), so the photon transmits unless the ReflectT draw is 0,
-                    // which it is exactly when the new MWC word is 0.
-                    if (mwc_next(sR, aR) != 0u) {
-                        tet = (int)(pk & kLinkMask);
-                        flags = (flags & ~F_STAGED) | ((((pk >> kBackShift) & 3u) + 1u) << kEntryShift);
-                    } else {
-                        const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
-                        const float two_ca1 = -2.0f * mincos;
-                        if (uz > 0.0f) flags |= F_NBS;
-                        ux = fmaf(two_ca1, pf.x, ux);
-                        uy = fmaf(two_ca1, pf.y, uy);
-                        uz = fmaf(two_ca1, pf.z, uz);
-                    }
-                } else {
-                    // COOP: the face's normal and neighbour word stay in the lane's stage.
-                    if (!COOP) {
-                        const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
-                        lds_face[threadIdx.x] = make_float4(pf.x, pf.y, pf.z, __uint_as_float(pk));
-                    }
-                    flags |= F_IFACE | ((uint32_t)imin << kFaceShift);
-                }
-            } else {
-                flags |= F_PEND;
+            maxz = (maxz < z) ? z : maxz;
+            // Packed neighbour word of the exit face: pool tet id (all ones at the mesh
+            // boundary), the shared face's index within the neighbour, and the neighbour's
+            // region. Selected on the bits of imin (-1 gives L.w, unused: no hit).
+            const bool i0 = (imin & 1) != 0, i1 = (imin & 2) != 0;
+            const uint32_t pk_lo = i0 ? L.y : L.x, pk_hi = i0 ? L.w : L.z;
+            const uint32_t pk = i1 ? pk_hi : pk_lo;
+            // A face into the same region (the region field equals the photon's and is not
+            // the escape, which the mesh boundary's all-ones word also carries) away from
+            // grazing incidence: FastReflectTransmit (src/octmps_kernel.cuh:590-805) with
+            // equal indices gives R = 0 (sam == 0), so the photon transmits unless the
+            // ReflectT draw is 0, which it is exactly when the new MWC word is 0.
+            const uint32_t field = pk >> kRegionShift;
+            const bool same = hit & (field == (uint32_t)region) & (field != kRegionEscape) & (-mincos >= kCos90F);
+            const uint64_t sRn = (sR & 0xffffffffull) * (uint64_t)aR + (sR >> 32);
+            sR = same ? sRn : sR;
+            const bool xmit = same & ((uint32_t)sRn != 0u);
+            tet = xmit ? (int)(pk & kLinkMask) : tet;
+            flags = xmit ? ((flags & ~F_STAGED) | ((((pk >> kBackShift) & 3u) + 1u) << kEntryShift)) : flags;
+            if (same & ((uint32_t)sRn == 0u)) {
+                const float4 pf_lo = i0 ? p1 : p0, pf_hi = i0 ? p3 : p2;
+                const float4 pf = i1 ? pf_hi : pf_lo;
+                const float two_ca1 = -2.0f * mincos;
+                if (uz > 0.0f) flags |= F_NBS;
+                ux = fmaf(two_ca1, pf.x, ux);
+                uy = fmaf(two_ca1, pf.y, uy);
+                uz = fmaf(two_ca1, pf.z, uz);
             }
+            // Any other face (another region, the mesh boundary, grazing incidence) parks the
+            // lane for the parked phase with its face index; COOP keeps the face's normal and
+            // neighbour word in the lane's stage, the per-lane path stashes them. A step that
+            // ends inside the tet parks the lane to scatter.
+            const bool iface = hit & !same;
+            if (!COOP && iface) {
+                const float4 pf_lo = i0 ? p1 : p0, pf_hi = i0 ? p3 : p2;
+                const float4 pf = i1 ? pf_hi : pf_lo;
+                lds_face[threadIdx.x] = make_float4(pf.x, pf.y, pf.z, __uint_as_float(pk));
+            }
+            flags |= iface ? (F_IFACE | ((uint32_t)imin << kFaceShift)) : (hit ? 0u : F_PEND);
+            crossed = same;
         }
         // Roulette after a step completed in the crossing phase.
-        if (ready && !finish && !(flags & (F_PEND | F_IFACE))) finish = roulette();
+        if (crossed && w < kWeightF) finish = roulette();
 
         // ---- Parked phase: interface events (Fresnel, reflection or transmission,
         // detection), or Drop + Spin/SpinBias, then roulette, for the parked lanes.
@@ -737,7 +789,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                     uy = ry;
                     uz = rz;
                 }
-                finish = roulette();
             } else if (flags & F_PEND) {
                 flags &= ~F_PEND;
                 // ---- Drop, src/octmps_kernel.cuh:584-588
@@ -749,79 +800,69 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
                     const float cost = hg_cos(RG.g, RG.hg_a, RG.hg_b, rand_oc(sS, aS));
                     rotate(ux, uy, uz, cost, rand_co(sS, aS), 0, ux, uy, uz);
                 } else if (tb == 37) {
-                    // SpinBias, src/octmps_kernel.cuh:898-1086
+                    // SpinBias, src/octmps_kernel.cuh:898-1086, written with selects: the
+                    // branch states (first bias, F = 1 or 2, selector) only choose values, and
+                    // the Spin stream's draws are taken in the reference's order (the selector
+                    // draw only when F != 0, then the cost draw, then psi).
                     const float ux0 = ux, uy0 = uy, uz0 = uz;
-                    int F = 0;
-                    bool first = false;
-                    float atmp = 0.0f;
-                    if (z > S.td_min && z < S.td_max && uz > 0.0f && !(flags & F_FST)) {
+                    // First backward bias: the continuation photon is copied before the flag
+                    // is set (a store, so this stays a branch; it happens once per photon).
+                    const bool first = (z > S.td_min) & (z < S.td_max) & (uz > 0.0f) & ((flags & F_FST) == 0);
+                    if (first) {
                         *reinterpret_cast<float4 *>(&C.x) = make_float4(x, y, z, w);
                         *reinterpret_cast<float4 *>(&C.ux) = make_float4(ux, uy, uz, op);
                         *reinterpret_cast<int4 *>(&C.maxz) = make_int4(__float_as_int(maxz), tet, region, 0);
-                        flags = (flags & ~F_CNBS) | ((flags & F_NBS) ? F_CNBS : 0u);
-                        F = 1;
-                        flags |= F_FST;
-                        loc_fst = z;
-                        atmp = S.bias_a;
-                        first = true;
                     }
+                    flags = first ? ((flags & ~F_CNBS) | ((flags & F_NBS) ? F_CNBS : 0u) | F_FST) : flags;
+                    loc_fst = first ? z : loc_fst;
                     const float4 probe = lds_probe_at();
                     float vx = probe.x - x, vy = probe.y - y, vz = probe.z - z;
                     const float len2 = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
-                    const float rl = rsqrtf(len2);
+                    const float rl = __builtin_amdgcn_rsqf(len2);
                     vx *= rl;
                     vy *= rl;
                     vz *= rl;
-                    if (((flags & F_FST) || (flags & F_NBS)) && F == 0) {
-                        F = 2;
-                        const float next = S.next_step_num * rcp(muas);
-                        const float dist = len2 * rl;  // |x - probe|
-                        if (next >= dist && -vz >= S.cos_angle) F = 1;
-                        atmp = S.bias_a;
-                    }
-                    int sel = 0;
-                    float cost, fx = ux, fy = uy, fz = uz;
-                    if (F) {
-                        sel = (rand_co(sS, aS) <= S.p_add) ? 1 : 2;
-                        if (F == 1 || sel == 1) {
-                            cost = bias_cos(atmp, S.inv_1ma, S.inv_sq, S.inv_2a, S.opa2, rand_oc(sS, aS));
-                            fx = vx;
-                            fy = vy;
-                            fz = vz;
-                        } else {
-                            cost = hg_cos(RG.g, RG.hg_a, RG.hg_b, rand_oc(sS, aS));
-                        }
-                    } else {
-                        cost = hg_cos(RG.g, RG.hg_a, RG.hg_b, rand_oc(sS, aS));
-                    }
-                    cost = fminf(fmaxf(cost, -1.0f), 1.0f);
+                    // A photon biased before, or reflected backwards: F = 2, or 1 when its next
+                    // step could reach the probe inside the acceptance cone.
+                    const bool later = !first & ((flags & (F_FST | F_NBS)) != 0);
+                    const float next = S.next_step_num * rcp(muas);
+                    const float dist = len2 * rl;  // |x - probe|
+                    const int F = first ? 1 : (later ? (((next >= dist) & (-vz >= S.cos_angle)) ? 1 : 2) : 0);
+                    const float atmp = (first | later) ? S.bias_a : 0.0f;
+                    const uint64_t sS1 = (sS & 0xffffffffull) * (uint64_t)aS + (sS >> 32);
+                    const float u_sel = unit_rz24((uint32_t)sS1);
+                    sS = (F != 0) ? sS1 : sS;
+                    const int sel = (F != 0) ? ((u_sel <= S.p_add) ? 1 : 2) : 0;
+                    const bool biased = (F == 1) | (sel == 1);
+                    const float u_c = rand_oc(sS, aS);
+                    const float cb = bias_cos(atmp, S.inv_1ma, S.inv_sq, S.inv_2a, S.opa2, u_c);
+                    const float ch = hg_cos(RG.g, RG.hg_a, RG.hg_b, u_c);
+                    const float cost = fminf(fmaxf(biased ? cb : ch, -1.0f), 1.0f);
+                    const float fx = biased ? vx : ux, fy = biased ? vy : uy, fz = biased ? vz : uz;
                     rotate(fx, fy, fz, cost, rand_co(sS, aS), 1, ux, uy, uz);
-                    if (atmp != 0.0f) {
-                        const float costg = fmaf(ux0, ux, fmaf(uy0, uy, uz0 * uz));
-                        const float g = RG.g;
-                        float factor;
-                        if (F == 1) {
-                            // Equation (8), src/octmps_kernel.cuh:1047-1050.
-                            const float t1 = fmaf(-2.0f * atmp, cost, S.opa2);
-                            const float t2 = fmaf(-2.0f * g, costg, fmaf(g, g, 1.0f));
-                            factor = RG.lr8 * (t1 * fsqrt(t1)) * rcp(t2 * fsqrt(t2));
-                        } else {
-                            // Equation (9), src/octmps_kernel.cuh:1071-1078.
-                            const float costg1 = fmaf(vx, ux, fmaf(vy, uy, vz * uz));
-                            const float k1 = sel == 1 ? cost : costg1;
-                            const float k2 = sel == 1 ? costg : cost;
-                            const float t1 = fmaf(-2.0f * atmp, k1, S.opa2);
-                            const float pdf1 = S.pdfb_k * rcp(t1 * fsqrt(t1));
-                            const float t2 = fsqrt(fmaf(-2.0f * g, k2, fmaf(g, g, 1.0f)));
-                            const float pdf2 = RG.hg_a * rcp(2.0f * (t2 * t2 * t2));
-                            factor = pdf2 * rcp(fmaf(S.p_add, pdf1, (1.0f - S.p_add) * pdf2));
-                        }
-                        lr *= factor;
-                        if (first) lr_fst = lr;
-                    }
+                    const float costg = fmaf(ux0, ux, fmaf(uy0, uy, uz0 * uz));
+                    const float g = RG.g;
+                    // Equation (8), src/octmps_kernel.cuh:1047-1050.
+                    const float e1 = fmaf(-2.0f * atmp, cost, S.opa2);
+                    const float e2 = fmaf(-2.0f * g, costg, fmaf(g, g, 1.0f));
+                    const float factor8 = RG.lr8 * (e1 * fsqrt(e1)) * rcp(e2 * fsqrt(e2));
+                    // Equation (9), src/octmps_kernel.cuh:1071-1078.
+                    const float costg1 = fmaf(vx, ux, fmaf(vy, uy, vz * uz));
+                    const float k1 = sel == 1 ? cost : costg1;
+                    const float k2 = sel == 1 ? costg : cost;
+                    const float t1 = fmaf(-2.0f * atmp, k1, S.opa2);
+                    const float pdf1 = S.pdfb_k * rcp(t1 * fsqrt(t1));
+                    const float t2 = fsqrt(fmaf(-2.0f * g, k2, fmaf(g, g, 1.0f)));
+                    const float pdf2 = RG.hg_a * rcp(2.0f * (t2 * t2 * t2));
+                    const float factor9 = pdf2 * rcp(fmaf(S.p_add, pdf1, (1.0f - S.p_add) * pdf2));
+                    // The likelihood ratio changes only for a biased state (atmp != 0).
+                    const bool apply = atmp != 0.0f;
+                    lr = apply ? lr * (F == 1 ? factor8 : factor9) : lr;
+                    lr_fst = (apply & first) ? lr : lr_fst;
                 }
-                finish = roulette();
             }
+            // Roulette (src/octmps_kernel.cuh:1376-1392) after the parked lanes' events.
+            if (parked && w < kWeightF) finish = roulette();
             // ComputeStepSize's draw for the next step of a scattered or restored photon,
             // made here: its next General draw is that one.
             if (parked && !finish && __float_as_uint(tau) == 0u) tau = neg_ln(rand_oc(sG, aG));
