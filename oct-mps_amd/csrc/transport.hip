@@ -570,137 +570,223 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             coop_fetch(ready && !(steps + 1u > max_steps) && !(flags & F_STAGED));
         }
 
-        // Step guard: a photon past max_steps is stopped and counted.
-        steps = ready ? steps + 1u : steps;
-        killed = ready && steps > max_steps;
-        finish = killed;
-        // Lanes whose step ended in this phase (a face into the same region), for roulette.
-        bool crossed = false;
-        // One divergent branch: every outcome inside is a select, since a divergent branch
-        // costs several scalar instructions and the loop is bound by instruction issue.
-        if (ready && !killed) {
-            // ---- ComputeStepSize, src/octmps_kernel.cuh:505-515. A fresh step length
-            // (sleft == 0) is normally drawn ahead, where the previous step ended (parked
-            // phase, launch), so this phase rarely runs the logarithm. +0 marks "draw now";
-            // a drawn length of zero is -0 (-ln(1)) and is used as is.
-            float tau_d = tau;
-            if (__float_as_uint(tau_d) == 0u) tau_d = neg_ln(rand_oc(sG, aG));
-            float s = tau_d * rmuas;
-            // ---- HitBoundary, src/octmps_kernel.cuh:517-572
-            // The tet record is read through a buffer resource (32-bit offsets). With
-            // SKIP, a photon that crossed into this tet through face k does not fetch
-            // plane k: that plane is the exact negation of the one it crossed, so its cosine
-            // is > 0 and the face is never a candidate. Its load is sent out of range instead,
-            // which costs no memory request and returns a zero plane (cosine 0, also never a
-            // candidate).
-            const uint32_t entry = (flags & F_ENTRY) >> kEntryShift;
-            flags &= ~F_ENTRY;
-            const uint32_t rec = (uint32_t)tet << 7;
-            auto plane_at = [&](uint32_t k) -> float4 {
-                const u4v v = __builtin_amdgcn_raw_buffer_load_b128(
-                    tets_rsrc, (SKIP && entry == k + 1u) ? kOutOfRange : rec, (int)(16u * k), 0);
-                return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
-                                   __uint_as_float(v.w));
-            };
-            float4 p0, p1, p2, p3;
-            uint4 L;
-            if (COOP) {
-                // The stage loads write LDS when they complete: wait for all of the wave's
-                // vector memory operations (s_waitcnt vmcnt(0)) before reading it.
+        // Crossing phase: one step of every ready lane. The two record paths get different
+        // control flow, each the faster one measured for it: with the cooperative fetch the
+        // records were loaded ahead and the phase is bound by instruction issue, so it runs
+        // as one divergent branch of selects; with per-lane gathers the phase issues its own
+        // loads and waits for them, and the branching form below runs faster.
+        if constexpr (COOP) {
+            // Step guard: a photon past max_steps is stopped and counted.
+            steps = ready ? steps + 1u : steps;
+            killed = ready && steps > max_steps;
+            finish = killed;
+            // Lanes whose step ended in this phase (a face into the same region), for roulette.
+            bool crossed = false;
+            // One divergent branch: every outcome inside is a select, since a divergent branch
+            // costs several scalar instructions and the loop is bound by instruction issue.
+            if (ready && !killed) {
+                // ---- ComputeStepSize, src/octmps_kernel.cuh:505-515. A fresh step length
+                // (sleft == 0) is normally drawn ahead, where the previous step ended (parked
+                // phase, launch), so this phase rarely runs the logarithm. +0 marks "draw now";
+                // a drawn length of zero is -0 (-ln(1)) and is used as is.
+                float tau_d = tau;
+                if (__float_as_uint(tau_d) == 0u) tau_d = neg_ln(rand_oc(sG, aG));
+                float s = tau_d * rmuas;
+                // ---- HitBoundary, src/octmps_kernel.cuh:517-572. The tet's four planes and
+                // links are in the lane's stage (fetched ahead); the stage loads write LDS when
+                // they complete: wait for all of the wave's vector memory operations (s_waitcnt
+                // vmcnt(0)) before reading it. The entry face is not needed here.
+                flags &= ~F_ENTRY;
                 __builtin_amdgcn_s_waitcnt(0x0F70);
                 asm volatile("" ::: "memory");
-                L = *own_links;
-                p0 = own_planes[0];
-                p1 = own_planes[1];
-                p2 = own_planes[2];
-                p3 = own_planes[3];
+                const uint4 L = *own_links;
+                const float4 p0 = own_planes[0], p1 = own_planes[1], p2 = own_planes[2], p3 = own_planes[3];
+                __builtin_amdgcn_sched_barrier(0);
+                const float c0 = fmaf(p0.z, uz, fmaf(p0.y, uy, p0.x * ux));
+                const float c1 = fmaf(p1.z, uz, fmaf(p1.y, uy, p1.x * ux));
+                const float c2 = fmaf(p2.z, uz, fmaf(p2.y, uy, p2.x * ux));
+                const float c3 = fmaf(p3.z, uz, fmaf(p3.y, uy, p3.x * ux));
+                float dmin = 1e10f, mincos = 0.0f;
+                int imin = -1;
+                // Faces in order with a strict '<', as the reference; written with selects so
+                // every plane is loaded whole and the wave does not branch per face.
+                auto face = [&](int I, const float4 &PL, float C) {
+                    const float perp = fmaf(PL.z, z, fmaf(PL.y, y, fmaf(PL.x, x, PL.w)));
+                    const float d = -perp * rcp(C);
+                    const bool take = (C < 0.0f) & (d < dmin);
+                    dmin = take ? d : dmin;
+                    imin = take ? I : imin;
+                    mincos = take ? C : mincos;
+                };
+                face(0, p0, c0);
+                face(1, p1, c1);
+                face(2, p2, c2);
+                face(3, p3, c3);
+                const bool hit = imin >= 0 && s > dmin;
+                // The leftover of a step that reaches a face, in optical depth. Glass (mus = 0)
+                // keeps the leftover unchanged, as (s - d) / DBL_MAX does in the reference; an
+                // overflowed step stays infinite, and a zero leftover is stored as +0 so that the
+                // next step draws.
+                const float left_m = (s - dmin) * muas;
+                const float left_g = __builtin_isinf(s) ? INFINITY : tau_d + 0.0f;
+                tau = hit ? (muas == 0.0f ? left_g : left_m) : 0.0f;
+                s = hit ? dmin : s;
+                // ---- Hop, src/octmps_kernel.cuh:574-582
+                x = fmaf(s, ux, x);
+                y = fmaf(s, uy, y);
+                z = fmaf(s, uz, z);
+                op += s;
+                maxz = (maxz < z) ? z : maxz;
+                // Packed neighbour word of the exit face: pool tet id (all ones at the mesh
+                // boundary), the shared face's index within the neighbour, and the neighbour's
+                // region. Selected on the bits of imin (-1 gives L.w, unused: no hit).
+                const bool i0 = (imin & 1) != 0, i1 = (imin & 2) != 0;
+                const uint32_t pk_lo = i0 ? L.y : L.x, pk_hi = i0 ? L.w : L.z;
+                const uint32_t pk = i1 ? pk_hi : pk_lo;
+                // A face into the same region (the region field equals the photon's and is not
+                // the escape, which the mesh boundary's all-ones word also carries) away from
+                // grazing incidence: FastReflectTransmit (src/octmps_kernel.cuh:590-805) with
+                // equal indices gives R = 0 (sam == 0), so the photon transmits unless the
+                // ReflectT draw is 0, which it is exactly when the new MWC word is 0.
+                const uint32_t field = pk >> kRegionShift;
+                const bool same = hit & (field == (uint32_t)region) & (field != kRegionEscape) & (-mincos >= kCos90F);
+                const uint64_t sRn = (sR & 0xffffffffull) * (uint64_t)aR + (sR >> 32);
+                sR = same ? sRn : sR;
+                const bool xmit = same & ((uint32_t)sRn != 0u);
+                tet = xmit ? (int)(pk & kLinkMask) : tet;
+                flags = xmit ? ((flags & ~F_STAGED) | ((((pk >> kBackShift) & 3u) + 1u) << kEntryShift)) : flags;
+                if (same & ((uint32_t)sRn == 0u)) {
+                    const float4 pf_lo = i0 ? p1 : p0, pf_hi = i0 ? p3 : p2;
+                    const float4 pf = i1 ? pf_hi : pf_lo;
+                    const float two_ca1 = -2.0f * mincos;
+                    if (uz > 0.0f) flags |= F_NBS;
+                    ux = fmaf(two_ca1, pf.x, ux);
+                    uy = fmaf(two_ca1, pf.y, uy);
+                    uz = fmaf(two_ca1, pf.z, uz);
+                }
+                // Any other face (another region, the mesh boundary, grazing incidence) parks the
+                // lane for the parked phase with its face index (the face's normal and neighbour
+                // word stay in the lane's stage). A step that ends inside the tet parks the lane
+                // to scatter.
+                const bool iface = hit & !same;
+                flags |= iface ? (F_IFACE | ((uint32_t)imin << kFaceShift)) : (hit ? 0u : F_PEND);
+                crossed = same;
+            }
+            // Roulette after a step completed in the crossing phase.
+            if (crossed && w < kWeightF) finish = roulette();
+        } else {
+            if (!ready) {
+                // parked or dead lane: nothing to do in the crossing phase
+            } else if (++steps > max_steps) {
+                finish = true;
+                killed = true;
             } else {
+                // ---- ComputeStepSize, src/octmps_kernel.cuh:505-515. A fresh step length
+                // (sleft == 0) is normally drawn ahead, where the previous step ended (parked
+                // phase, launch), so this phase rarely runs the logarithm. +0 marks "draw now";
+                // a drawn length of zero is -0 (-ln(1)) and is used as is.
+                float tau_d = tau;
+                if (__float_as_uint(tau_d) == 0u) tau_d = neg_ln(rand_oc(sG, aG));
+                float s = tau_d * rmuas;
+                tau = 0.0f;
+                // ---- HitBoundary, src/octmps_kernel.cuh:517-572
+                // The tet record is read through a buffer resource (32-bit offsets). With
+                // SKIP, a photon that crossed into this tet through face k does not fetch
+                // plane k: that plane is the exact negation of the one it crossed, so its cosine
+                // is > 0 and the face is never a candidate. Its load is sent out of range instead,
+                // which costs no memory request and returns a zero plane (cosine 0, also never a
+                // candidate).
+                const uint32_t entry = (flags & F_ENTRY) >> kEntryShift;
+                flags &= ~F_ENTRY;
+                const uint32_t rec = (uint32_t)tet << 7;
+                auto plane_at = [&](uint32_t k) -> float4 {
+                    const u4v v = __builtin_amdgcn_raw_buffer_load_b128(
+                        tets_rsrc, (SKIP && entry == k + 1u) ? kOutOfRange : rec, (int)(16u * k), 0);
+                    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                                       __uint_as_float(v.w));
+                };
+                float4 p0, p1, p2, p3;
+                uint4 L;
                 p0 = plane_at(0);
                 p1 = plane_at(1);
                 p2 = plane_at(2);
                 p3 = plane_at(3);
-                const u4v lv = __builtin_amdgcn_raw_buffer_load_b128(tets_rsrc, rec + 64u, 0, 0);
-                L = make_uint4(lv.x, lv.y, lv.z, lv.w);
+                {
+                    const u4v lv = __builtin_amdgcn_raw_buffer_load_b128(tets_rsrc, rec + 64u, 0, 0);
+                    L = make_uint4(lv.x, lv.y, lv.z, lv.w);
+                }
+                // Keep the loads together ahead of the face tests (one memory round trip).
+                __builtin_amdgcn_sched_barrier(0);
+                const float c0 = fmaf(p0.z, uz, fmaf(p0.y, uy, p0.x * ux));
+                const float c1 = fmaf(p1.z, uz, fmaf(p1.y, uy, p1.x * ux));
+                const float c2 = fmaf(p2.z, uz, fmaf(p2.y, uy, p2.x * ux));
+                const float c3 = fmaf(p3.z, uz, fmaf(p3.y, uy, p3.x * ux));
+                float dmin = 1e10f, mincos = 0.0f;
+                int imin = -1;
+                // Faces in order with a strict '<', as the reference; written with selects so
+                // every plane is loaded whole and the wave does not branch per face.
+                auto face = [&](int I, const float4 &PL, float C) {
+                    const float perp = fmaf(PL.z, z, fmaf(PL.y, y, fmaf(PL.x, x, PL.w)));
+                    const float d = -perp * rcp(C);
+                    const bool take = (C < 0.0f) & (d < dmin);
+                    dmin = take ? d : dmin;
+                    imin = take ? I : imin;
+                    mincos = take ? C : mincos;
+                };
+                face(0, p0, c0);
+                face(1, p1, c1);
+                face(2, p2, c2);
+                face(3, p3, c3);
+                const bool hit = imin >= 0 && s > dmin;
+                if (hit) {
+                    // Glass (mus = 0) keeps the leftover unchanged, as (s - d) / DBL_MAX does
+                    // in the reference; an overflowed step stays infinite, and a zero leftover
+                    // is stored as +0 so that the next step draws.
+                    tau = (muas == 0.0f) ? (isinf(s) ? INFINITY : tau_d + 0.0f) : (s - dmin) * muas;
+                    s = dmin;
+                }
+                // ---- Hop, src/octmps_kernel.cuh:574-582
+                x = fmaf(s, ux, x);
+                y = fmaf(s, uy, y);
+                z = fmaf(s, uz, z);
+                op += s;
+                if (maxz < z) maxz = z;
+
+                if (hit) {
+                    // Packed neighbour word: pool tet id (all ones at the mesh boundary), the
+                    // shared face's index within the neighbour, and the neighbour's region.
+                    const uint32_t pk = imin == 0 ? L.x : (imin == 1 ? L.y : (imin == 2 ? L.z : L.w));
+                    // Same region: the region field equals the photon's and is not the escape
+                    // (which the mesh boundary's all-ones word also carries).
+                    const uint32_t field = pk >> kRegionShift;
+                    if (field == (uint32_t)region && field != kRegionEscape && -mincos >= kCos90F) {
+                        // FastReflectTransmit (src/octmps_kernel.cuh:590-805) on a face into the
+                        // same region away from grazing incidence: equal indices give R = 0
+                        // (sam == 0), so the photon transmits unless the ReflectT draw is 0,
+                        // which it is exactly when the new MWC word is 0.
+                        if (mwc_next(sR, aR) != 0u) {
+                            tet = (int)(pk & kLinkMask);
+                            flags = (flags & ~F_STAGED) | ((((pk >> kBackShift) & 3u) + 1u) << kEntryShift);
+                        } else {
+                            const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
+                            const float two_ca1 = -2.0f * mincos;
+                            if (uz > 0.0f) flags |= F_NBS;
+                            ux = fmaf(two_ca1, pf.x, ux);
+                            uy = fmaf(two_ca1, pf.y, uy);
+                            uz = fmaf(two_ca1, pf.z, uz);
+                        }
+                    } else {
+                        const float4 pf = imin == 0 ? p0 : (imin == 1 ? p1 : (imin == 2 ? p2 : p3));
+                        lds_face[threadIdx.x] = make_float4(pf.x, pf.y, pf.z, __uint_as_float(pk));
+                        flags |= F_IFACE | ((uint32_t)imin << kFaceShift);
+                    }
+                } else {
+                    flags |= F_PEND;
+                }
             }
-            // Keep the loads together ahead of the face tests (one memory round trip).
-            __builtin_amdgcn_sched_barrier(0);
-            const float c0 = fmaf(p0.z, uz, fmaf(p0.y, uy, p0.x * ux));
-            const float c1 = fmaf(p1.z, uz, fmaf(p1.y, uy, p1.x * ux));
-            const float c2 = fmaf(p2.z, uz, fmaf(p2.y, uy, p2.x * ux));
-            const float c3 = fmaf(p3.z, uz, fmaf(p3.y, uy, p3.x * ux));
-            float dmin = 1e10f, mincos = 0.0f;
-            int imin = -1;
-            // Faces in order with a strict '<', as the reference; written with selects so
-            // every plane is loaded whole and the wave does not branch per face.
-            auto face = [&](int I, const float4 &PL, float C) {
-                const float perp = fmaf(PL.z, z, fmaf(PL.y, y, fmaf(PL.x, x, PL.w)));
-                const float d = -perp * rcp(C);
-                const bool take = (C < 0.0f) & (d < dmin);
-                dmin = take ? d : dmin;
-                imin = take ? I : imin;
-                mincos = take ? C : mincos;
-            };
-            face(0, p0, c0);
-            face(1, p1, c1);
-            face(2, p2, c2);
-            face(3, p3, c3);
-            const bool hit = imin >= 0 && s > dmin;
-            // The leftover of a step that reaches a face, in optical depth. Glass (mus = 0)
-            // keeps the leftover unchanged, as (s - d) / DBL_MAX does in the reference; an
-            // overflowed step stays infinite, and a zero leftover is stored as +0 so that the
-            // next step draws.
-            const float left_m = (s - dmin) * muas;
-            const float left_g = __builtin_isinf(s) ? INFINITY : tau_d + 0.0f;
-            tau = hit ? (muas == 0.0f ? left_g : left_m) : 0.0f;
-            s = hit ? dmin : s;
-            // ---- Hop, src/octmps_kernel.cuh:574-582
-            x = fmaf(s, ux, x);
-            y = fmaf(s, uy, y);
-            z = fmaf(s, uz, z);
-            op += s;
-            maxz = (maxz < z) ? z : maxz;
-            // Packed neighbour word of the exit face: pool tet id (all ones at the mesh
-            // boundary), the shared face's index within the neighbour, and the neighbour's
-            // region. Selected on the bits of imin (-1 gives L.w, unused: no hit).
-            const bool i0 = (imin & 1) != 0, i1 = (imin & 2) != 0;
-            const uint32_t pk_lo = i0 ? L.y : L.x, pk_hi = i0 ? L.w : L.z;
-            const uint32_t pk = i1 ? pk_hi : pk_lo;
-            // A face into the same region (the region field equals the photon's and is not
-            // the escape, which the mesh boundary's all-ones word also carries) away from
-            // grazing incidence: FastReflectTransmit (src/octmps_kernel.cuh:590-805) with
-            // equal indices gives R = 0 (sam == 0), so the photon transmits unless the
-            // ReflectT draw is 0, which it is exactly when the new MWC word is 0.
-            const uint32_t field = pk >> kRegionShift;
-            const bool same = hit & (field == (uint32_t)region) & (field != kRegionEscape) & (-mincos >= kCos90F);
-            const uint64_t sRn = (sR & 0xffffffffull) * (uint64_t)aR + (sR >> 32);
-            sR = same ? sRn : sR;
-            const bool xmit = same & ((uint32_t)sRn != 0u);
-            tet = xmit ? (int)(pk & kLinkMask) : tet;
-            flags = xmit ? ((flags & ~F_STAGED) | ((((pk >> kBackShift) & 3u) + 1u) << kEntryShift)) : flags;
-            if (same & ((uint32_t)sRn == 0u)) {
-                const float4 pf_lo = i0 ? p1 : p0, pf_hi = i0 ? p3 : p2;
-                const float4 pf = i1 ? pf_hi : pf_lo;
-                const float two_ca1 = -2.0f * mincos;
-                if (uz > 0.0f) flags |= F_NBS;
-                ux = fmaf(two_ca1, pf.x, ux);
-                uy = fmaf(two_ca1, pf.y, uy);
-                uz = fmaf(two_ca1, pf.z, uz);
-            }
-            // Any other face (another region, the mesh boundary, grazing incidence) parks the
-            // lane for the parked phase with its face index; COOP keeps the face's normal and
-            // neighbour word in the lane's stage, the per-lane path stashes them. A step that
-            // ends inside the tet parks the lane to scatter.
-            const bool iface = hit & !same;
-            if (!COOP && iface) {
-                const float4 pf_lo = i0 ? p1 : p0, pf_hi = i0 ? p3 : p2;
-                const float4 pf = i1 ? pf_hi : pf_lo;
-                lds_face[threadIdx.x] = make_float4(pf.x, pf.y, pf.z, __uint_as_float(pk));
-            }
-            flags |= iface ? (F_IFACE | ((uint32_t)imin << kFaceShift)) : (hit ? 0u : F_PEND);
-            crossed = same;
+            // Roulette after a step completed in the crossing phase.
+            if (ready && !finish && !(flags & (F_PEND | F_IFACE))) finish = roulette();
         }
-        // Roulette after a step completed in the crossing phase.
-        if (crossed && w < kWeightF) finish = roulette();
 
         // ---- Parked phase: interface events (Fresnel, reflection or transmission,
         // detection), or Drop + Spin/SpinBias, then roulette, for the parked lanes.
