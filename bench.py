@@ -28,22 +28,26 @@ LAYOUT_BYTES_PER_STEP = 80  # this build's packed tet record: 4 planes x 16 B + 
 # Counter-derived figures of the batched slab4 launch (128 A-scans x 1e6 photons) for each record
 # path, each from the rocprofv3 summaries named beside it; per photon-step = counter / the
 # launch's photon-steps. HBM bytes = FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors by which
-# those counters read a copy kernel of known size).
+# those counters read a copy kernel of known size). Wave-cycle shares are SQ_ACTIVE_INST_ANY
+# (issuing), SQ_WAIT_INST_ANY (ready, waiting for an issue slot) and SQ_WAIT_ANY (waiting on a
+# dependency) over SQ_WAVE_CYCLES.
 PROFILE = {
     # Cooperative record fetch, quanta 28/1.
-    "coop": {"l1_lookups_per_step": 1.72, "ta_busy": 0.72, "tcp_pending_l2": 0.45, "l2_hit": 0.72,
-             "l2_latency_cycles": 308, "wave_wait_dependency": 0.35, "wave_wait_issue": 0.28, "wave_issuing": 0.37,
-             "valu_per_step": 4.95, "hbm_bytes_per_step": 25.01,
+    "coop": {"l1_lookups_per_step": 1.72, "ta_busy": 0.64, "tcp_pending_l2": 0.39, "l2_hit": 0.717,
+             "l2_latency_cycles": 323, "wave_wait_dependency": 0.32, "wave_wait_issue": 0.34, "wave_issuing": 0.34,
+             "valu_per_step": 4.98, "salu_per_step": 1.74, "salu_per_cu_cycle": 0.464, "hbm_bytes_per_step": 24.98,
              "source": "profiles/transport_slab4_memory.md, transport_slab4_sq.md, transport_slab4_pmc.md"},
     # Per-lane gathers, quanta 24/20, entry-plane skip on / off.
-    "lane_skip_on": {"l1_lookups_per_step": 4.16, "ta_busy": 0.77, "tcp_pending_l2": 0.15, "valu_per_step": 5.75,
-                     "hbm_bytes_per_step": 25.58, "source": "profiles/transport_slab4_perlane_skip_counters.md"},
-    "lane_skip_off": {"l1_lookups_per_step": 5.01, "ta_busy": 0.85, "tcp_pending_l2": 0.01, "valu_per_step": 5.47,
-                      "hbm_bytes_per_step": 25.58,
+    "lane_skip_on": {"l1_lookups_per_step": 4.16, "ta_busy": 0.84, "tcp_pending_l2": 0.18, "valu_per_step": 5.59,
+                     "salu_per_step": 2.56, "salu_per_cu_cycle": 0.475, "hbm_bytes_per_step": 25.53,
+                     "source": "profiles/transport_slab4_perlane_skip_memory.md, transport_slab4_perlane_skip_sq.md, "
+                               "transport_slab4_perlane_skip_pmc.md"},
+    "lane_skip_off": {"l1_lookups_per_step": 5.01, "ta_busy": 0.87, "tcp_pending_l2": 0.01, "valu_per_step": 5.32,
+                      "salu_per_step": 2.53, "salu_per_cu_cycle": 0.485, "hbm_bytes_per_step": 25.56,
                       "source": "profiles/transport_slab4_perlane_memory.md, transport_slab4_perlane_sq.md, "
                                 "transport_slab4_perlane_pmc.md"},
     # The bump sample (64 of 1024 A-scans x 1e6 photons, cooperative fetch, quanta 32/16).
-    "bump_hbm_bytes_per_step": 44.05, "bump_hbm_source": "profiles/transport_bump_pmc.md",
+    "bump_hbm_bytes_per_step": 44.09, "bump_hbm_source": "profiles/transport_bump_pmc.md",
 }
 
 
@@ -366,11 +370,13 @@ def main():
     # Ceiling of the vector-memory path for divergent record gathers (cache-resident record
     # set), and the measured HBM copy rate beside the nominal peak the roofline uses: 4 GiB
     # read + 4 GiB written per copy (16x the Infinity Cache).
-    gather_peak = hbm_copy = l1_probe = None
+    gather_peak = hbm_copy = l1_probe = valu_probe = None
     if not a.no_probes:
         gather_peak = ctx.probe_gather()
         l1_probe = {name: ctx.probe_l1(i)[1] for i, name in enumerate(
             ("divergent", "quad_coalesced", "half_masked", "half_out_of_range", "quad_coalesced_lds_dma"))}
+        valu_probe = {name: ctx.probe_valu(i) for i, name in enumerate(("fma_f32", "rcp_f32", "mwc_step"))}
+        log("VALU probe, SIMD cycles per wave instruction: " + ", ".join(f"{k} {v[1]:.2f}" for k, v in valu_probe.items()))
         log("L1 probe, CU cycles per 16-byte wave load: " + ", ".join(f"{k} {v:.1f}" for k, v in l1_probe.items()))
         nb = 4 << 30
         ms = ctx.probe_copy(nb, 5)
@@ -452,6 +458,30 @@ def main():
                   "l1_probe_cu_cycles_per_wave_load": l1_probe}
     ctx.close()
     frac = achieved / HBM_PEAK_GBS
+    if path == "coop":
+        # Records are loaded ahead of use, so the loop is bound by issuing its instructions: the
+        # share of each wave's cycles spent issuing or ready and waiting for an issue slot,
+        # against waiting on data (SQ counters of the profiled launch).
+        bound_by = {"resource": "instruction issue (VALU and SALU instructions and L1 lookups together; waiting on "
+                                "memory is the smaller share)",
+                    "frac": prof["wave_issuing"] + prof["wave_wait_issue"],
+                    "frac_of": "share of each wave's cycles spent issuing or ready and waiting for an issue slot "
+                               "(SQ_ACTIVE_INST_ANY + SQ_WAIT_INST_ANY over SQ_WAVE_CYCLES); the rest, "
+                               f"{prof['wave_wait_dependency']:.2f}, waits on a dependency"}
+    else:
+        bound_by = {"resource": "L1 lookups of the per-lane record gathers (one per lane per 16-byte load)",
+                    "frac": gather["frac"] if gather else None,
+                    "frac_of": "L1 lookups per second over the divergent-gather ceiling of octmps_probe_gather"}
+    bound_by.update({"record_path": path, "counters": {k: v for k, v in prof.items() if k != "source"},
+                     "source": prof["source"], "lane_utilisation": lane_util, "gather_frac": gather["frac"] if gather else None})
+    if valu_probe:
+        # VALU instructions the launch retired per second, against a stream of independent FP32
+        # FMAs (octmps_probe_valu): above 1 means most of its VALU instructions issue in fewer
+        # SIMD cycles than an FMA; the probe's cycle costs per class are listed beside it.
+        valu_rate = prof["valu_per_step"] * sum(steps) / kernel_s
+        bound_by["valu"] = {"insts_per_s": valu_rate, "fma_stream_insts_per_s": valu_probe["fma_f32"][0],
+                            "ratio_to_fma_stream": valu_rate / valu_probe["fma_f32"][0],
+                            "probe_simd_cycles_per_inst": {k: v[1] for k, v in valu_probe.items()}}
     line = {
         "metric": "photons/s per B-scan, batched vs plain kernel; p50/p99 A-scan request latency",
         "value": value,
@@ -489,13 +519,7 @@ def main():
                      "layout_bytes_per_step": LAYOUT_BYTES_PER_STEP,
                      "measured_hbm": {"bytes_per_step": prof["hbm_bytes_per_step"], "GB_s": hbm_measured,
                                       "frac": hbm_measured / HBM_PEAK_GBS, "source": prof["source"]}},
-        "bound_by": {"resource": ("latency of L1 misses: the L1 waits on L2 data for much of the launch, each "
-                                  "wave waits on a dependency a third of its time, at 6 waves per SIMD"
-                                  if path == "coop" else
-                                  "L1 lookups of the per-lane record gathers (one per lane per 16-byte load)"),
-                     "record_path": path, "gather_frac": gather["frac"] if gather else None,
-                     "counters": {k: v for k, v in prof.items() if k != "source"}, "source": prof["source"],
-                     "lane_utilisation": lane_util},
+        "bound_by": bound_by,
         "gather_roofline": gather,
         "bump_sample": bump,
         "cpu_baseline": cpu,

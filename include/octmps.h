@@ -438,6 +438,15 @@ octmps_status octmps_probe_gather(octmps_ctx *ctx, int32_t n_records, int32_t it
 octmps_status octmps_probe_l1(octmps_ctx *ctx, int32_t pattern, int32_t iters, double *loads_per_s,
                               double *cu_cycles_per_load);
 
+/* VALU issue rate by instruction class (no counterpart in the reference): every lane of a
+ * 6-waves-per-SIMD grid runs iters x 8 independent operations of one class. kind 0: FP32
+ * fused multiply-add; 1: FP32 reciprocal (the transcendental class); 2: an MWC step (64-bit
+ * integer multiply-add and a move: two instructions, counted as one). Returns wave-level instructions per second over the GPU and
+ * SIMD cycles per instruction at the nominal clock: the denominators of the transport
+ * kernel's VALU issue fraction. */
+octmps_status octmps_probe_valu(octmps_ctx *ctx, int32_t kind, int32_t iters, double *insts_per_s,
+                                double *simd_cycles_per_inst);
+
 #ifdef __cplusplus
 }
 #endif

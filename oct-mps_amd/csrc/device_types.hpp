@@ -201,5 +201,7 @@ hipError_t launch_gather_probe(const TetRec *recs, uint32_t n, int iters, uint32
                                hipStream_t stream);
 // blocks x 256 lanes each issue iters x 8 loads of `pattern` from a 16 KiB buffer (buf).
 hipError_t launch_l1_probe(int pattern, const uint8_t *buf, int iters, float *out, int blocks, hipStream_t stream);
+// blocks x 256 lanes each run iters x 8 independent VALU operations of class `kind`.
+hipError_t launch_valu_probe(int kind, int iters, float *out, int blocks, hipStream_t stream);
 
 }  // namespace octmps

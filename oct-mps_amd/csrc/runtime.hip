@@ -690,6 +690,33 @@ extern "C" octmps_status octmps_probe_gather(octmps_ctx *c, int32_t n_records, i
     return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_gather");
 }
 
+extern "C" octmps_status octmps_probe_valu(octmps_ctx *c, int32_t kind, int32_t iters, double *insts_per_s,
+                                          double *simd_cycles_per_inst) {
+    if (!c || kind < 0 || kind > 2 || iters <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");
+    std::lock_guard<std::mutex> lk(c->run_mu);
+    if (hipError_t se = hipSetDevice(c->device); se != hipSuccess) return hip_fail(se, "set device");
+    const int blocks = c->n_cu * 6;  // 6 waves per SIMD, as the transport kernel
+    int khz = 0;
+    hipError_t e = hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, c->device);
+    if (e != hipSuccess) return hip_fail(e, "clock rate");
+    void *out = nullptr;
+    if ((e = hipMalloc(&out, (size_t)blocks * 256 * sizeof(float))) != hipSuccess) return hip_fail(e, "probe allocation");
+    e = launch_valu_probe(kind, 16, (float *)out, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess) e = launch_valu_probe(kind, iters, (float *)out, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    float ms = 0;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(out);
+    const double insts = (double)blocks * 4.0 * iters * 8.0;  // wave-level instructions of the class
+    const double rate = ms > 0 ? insts / (ms * 1e-3) : 0.0;
+    if (insts_per_s) *insts_per_s = rate;
+    if (simd_cycles_per_inst) *simd_cycles_per_inst = rate > 0 ? c->n_cu * 4.0 * (khz * 1e3) / rate : 0.0;
+    return e == hipSuccess ? OCTMPS_OK : hip_fail(e, "probe_valu");
+}
+
 extern "C" octmps_status octmps_probe_l1(octmps_ctx *c, int32_t pattern, int32_t iters, double *loads_per_s,
                                         double *cu_cycles_per_load) {
     if (!c || pattern < 0 || pattern > 4 || iters <= 0) return fail(OCTMPS_E_INVALID, "bad probe arguments");

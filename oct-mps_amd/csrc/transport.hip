@@ -1243,7 +1243,47 @@ __global__ void __launch_bounds__(256) l1_probe_kernel(const uint8_t *buf, int i
     }
     out[blockIdx.x * 256u + threadIdx.x] = acc;
 }
+
+// VALU issue rate by instruction class: every lane runs iters x 8 independent operations on
+// eight accumulators (no dependency between consecutive instructions of a wave), at 6 waves
+// per SIMD. Class 0: v_fma_f32 (the common FP32 operation); 1: v_rcp_f32 (the transcendental
+// class: reciprocal, square root, log, sin and cos); 2: one MWC step (v_mad_u64_u32 and the
+// move of the previous high word into its addend, the two instructions the transport kernel
+// spends per draw).
+template <int K>
+__global__ void __launch_bounds__(256) valu_probe_kernel(int iters, float *out) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[8];
+    uint64_t b[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        a[k] = 1.0f + 1e-3f * (float)((g + 7u * k) & 255u);
+        b[k] = ((uint64_t)(g + k) << 32) | (0x9E3779B9u ^ (uint32_t)k);
+    }
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (K == 0) a[k] = fmaf(a[k], 0.999f, 1e-3f);
+            if (K == 1) a[k] = rcp(a[k]);
+            if (K == 2) b[k] = (b[k] & 0xffffffffull) * 4294967118ull + (b[k] >> 32);
+        }
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += a[k] + (float)(uint32_t)b[k];
+    out[g] = acc;
+}
 }  // namespace
+
+hipError_t launch_valu_probe(int kind, int iters, float *out, int blocks, hipStream_t stream) {
+    switch (kind) {
+    case 0: hipLaunchKernelGGL(valu_probe_kernel<0>, dim3(blocks), dim3(256), 0, stream, iters, out); break;
+    case 1: hipLaunchKernelGGL(valu_probe_kernel<1>, dim3(blocks), dim3(256), 0, stream, iters, out); break;
+    case 2: hipLaunchKernelGGL(valu_probe_kernel<2>, dim3(blocks), dim3(256), 0, stream, iters, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_l1_probe(int pattern, const uint8_t *buf, int iters, float *out, int blocks, hipStream_t stream) {
     switch (pattern) {

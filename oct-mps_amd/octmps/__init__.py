@@ -194,6 +194,7 @@ SIGNATURES = [
     ("octmps_probe_copy", C.c_int, [_P, C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
     ("octmps_probe_gather", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     ("octmps_probe_l1", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("octmps_probe_valu", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 
 
@@ -520,6 +521,14 @@ class Context:
         4 quad-coalesced DMA into LDS."""
         r, cyc = C.c_double(0), C.c_double(0)
         check(lib.octmps_probe_l1(self.h, pattern, iters, C.byref(r), C.byref(cyc)), "probe_l1")
+        return r.value, cyc.value
+
+    def probe_valu(self, kind, iters=4096):
+        """(wave VALU instructions/s, SIMD cycles per instruction at the nominal clock) of an
+        instruction class: 0 FP32 fused multiply-add, 1 FP32 reciprocal, 2 64-bit integer
+        multiply-add."""
+        r, cyc = C.c_double(0), C.c_double(0)
+        check(lib.octmps_probe_valu(self.h, kind, iters, C.byref(r), C.byref(cyc)), "probe_valu")
         return r.value, cyc.value
 
     def probe_sampler(self, kind, param, u):

@@ -310,6 +310,19 @@ def test_l1_probe_shows_the_lookup_cost_of_each_pattern(ctx):
         ctx.probe_l1(5)
 
 
+def test_valu_probe_rates_by_instruction_class(ctx):
+    # Issue cost of the VALU classes the transport loop uses, at 6 waves per SIMD: a wave64
+    # FP32 FMA takes a few SIMD cycles, a transcendental at least as many, an MWC step (two
+    # instructions) more than one FMA.
+    rates = [ctx.probe_valu(k) for k in range(3)]
+    for r, cyc in rates:
+        assert r > 0 and 0.5 < cyc < 64
+    assert rates[1][1] >= 0.9 * rates[0][1]
+    assert rates[2][1] > rates[0][1]
+    with pytest.raises(O.OctmpsError):
+        ctx.probe_valu(3)
+
+
 def test_autotune_picks_a_valid_configuration(ctx, small_meshes):
     _, mid, p, s, seed = small_meshes["slab"]
     o, rate = ctx.autotune(mid, p, s, seed, calib=4000)
