@@ -5,7 +5,7 @@ host cores beside it.
 
 A step is one full B-scan (all A-scans of the workload) through the batched path, launched
 as one hipGraph. The whole run keeps to a wall-clock budget (--budget-s): the fixed-cost
-legs run first (autotune on a 1/64 slice, warmup steps on a 1/64 slice, the plain path on
+legs run first (autotune on a 1/64 slice of at least 4e6 photons, warmup steps on a 1/64 slice, the plain path on
 every 8th A-scan, the CPU oracle on a bounded sample, the served stream and its load sweep,
 the cache and HBM probes), then as many of the requested timed steps as fit in what is left
 (at least one). `steps` reports the number timed; `steps_requested` the number asked for.
@@ -65,7 +65,8 @@ def parse():
     ap.add_argument("--scans", type=int, default=128, help="A-scans in the B-scan (workload: 128)")
     ap.add_argument("--photons", type=int, default=1_000_000, help="photons per A-scan (workload: 1e6)")
     ap.add_argument("--slice-div", type=int, default=64,
-                    help="warmup and autotune calibration use 1/k of every A-scan's photons")
+                    help="warmup and autotune calibration use 1/k of every A-scan's photons (autotune: at least 4e6 "
+                         "photons in all)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-photons", type=int, default=3000, help="oracle photons per sampled A-scan")
@@ -261,7 +262,9 @@ def main():
         # The autotuner (coordinate search over launch options on a calibration slice of the
         # same B-scan: the first 1/div of every A-scan's photons), before the timed region;
         # results are the same bits under any options.
-        calib = max(1, total_photons // div)
+        # At least 4e6 photons (~10 per lane of the grid): a smaller slice is mostly the launch's
+        # tail, and options tuned on a tail lose on the full B-scan.
+        calib = min(total_photons, max(total_photons // div, 4_000_000))
         ts = time.perf_counter()
         o, rate = ctx.autotune(mid, params, scans, seed, calib=calib)
         opts.block_size, opts.refill_quantum, opts.scatter_quantum = o.block_size, o.refill_quantum, o.scatter_quantum

@@ -326,7 +326,9 @@ octmps_status octmps_run_bscan_logged(octmps_ctx *ctx, int32_t mesh_id, const oc
  * writes the winner into *opts (scatter_quantum, block_size, refill_quantum, tally_copies,
  * entry_skip, record_fetch), makes the first four the context's default (entry_skip and
  * record_fetch stay per launch, since their best values depend on the mesh), and returns its
- * photons/s in *rate. */
+ * photons/s in *rate. The slice is the first calib_photons / n_scans photons of every A-scan,
+ * raised to at least 8 photons per lane of a full-occupancy grid (about 3.1e6 on 256 CUs)
+ * where the B-scan holds that many: a smaller slice is mostly the launch's tail. */
 octmps_status octmps_autotune(octmps_ctx *ctx, int32_t mesh_id, const octmps_params *p,
                               int32_t n_scans, const octmps_ascan *scans, uint64_t seed,
                               uint64_t calib_photons, octmps_batch_opts *opts, double *rate);

@@ -468,7 +468,16 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
                                          octmps_batch_opts *opts, double *rate) {
     if (!c || !p || !scans || n <= 0 || !opts) return fail(OCTMPS_E_INVALID, "null argument");
     if (calib == 0) calib = 1u << 20;
-    // Calibration slice: the first calib/n photons of every A-scan.
+    // Calibration slice: the first calib/n photons of every A-scan, but at least 8 photons per
+    // lane of a full-occupancy grid (6 blocks of 256 lanes per CU) where the B-scan has them: a
+    // slice of fewer photons than that is mostly the launch's tail (the last, longest photons
+    // on a nearly idle GPU), and options tuned on a tail lose on the full B-scan.
+    {
+        uint64_t total = 0;
+        for (int32_t i = 0; i < n; ++i) total += scans[i].n_photons;
+        const uint64_t floor_photons = (uint64_t)c->n_cu * 6u * 256u * 8u;
+        calib = std::max<uint64_t>(calib, std::min<uint64_t>(total, floor_photons));
+    }
     std::vector<octmps_ascan> cs(scans, scans + n);
     uint64_t per = std::max<uint64_t>(1, calib / (uint64_t)n);
     for (auto &s : cs) s.n_photons = std::min<uint64_t>(s.n_photons, per);
