@@ -46,8 +46,9 @@ PROFILE = {
                       "salu_per_step": 2.53, "salu_per_cu_cycle": 0.485, "hbm_bytes_per_step": 25.56,
                       "source": "profiles/transport_slab4_perlane_memory.md, transport_slab4_perlane_sq.md, "
                                 "transport_slab4_perlane_pmc.md"},
-    # The bump sample (64 of 1024 A-scans x 1e6 photons, cooperative fetch, quanta 32/16).
-    "bump_hbm_bytes_per_step": 44.09, "bump_hbm_source": "profiles/transport_bump_pmc.md",
+    # The bump sample (64 of 1024 A-scans x 1e6 photons, cooperative fetch, quanta 28/1, 4 blocks
+    # per CU).
+    "bump_hbm_bytes_per_step": 43.99, "bump_hbm_source": "profiles/transport_bump_pmc.md",
 }
 
 
@@ -80,6 +81,11 @@ def parse():
                     help="with --quanta: entry-plane skip of the per-lane record path (0 auto, 1 on, 2 off)")
     ap.add_argument("--record-fetch", type=int, default=-1, choices=(-1, 0, 1, 2),
                     help="with --quanta: record path (0 auto, 1 per-lane gathers, 2 cooperative)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0,
+                    help="with --quanta: resident blocks per CU (0: the occupancy maximum)")
+    ap.add_argument("--bump-quanta", default="28,1,4",
+                    help="scatter quantum, refill quantum and blocks per CU of the bump sample (the autotuner's "
+                         "choice for that mesh)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate of the C3 stream, req/s")
     ap.add_argument("--serve-window", type=float, default=0.1, help="seconds of arrivals in the C3 stream")
     ap.add_argument("--sweep-rates", default="10,25,50,100,200",
@@ -256,8 +262,10 @@ def main():
             opts.entry_skip = a.entry_skip
         if a.record_fetch >= 0:
             opts.record_fetch = a.record_fetch
+        opts.blocks_per_cu = a.blocks_per_cu
         tuned = {"scatter_quantum": opts.scatter_quantum, "refill_quantum": opts.refill_quantum,
-                 "entry_skip": opts.entry_skip, "record_fetch": opts.record_fetch, "source": "--quanta"}
+                 "entry_skip": opts.entry_skip, "record_fetch": opts.record_fetch,
+                 "blocks_per_cu": opts.blocks_per_cu, "source": "--quanta"}
     elif not a.no_autotune:
         # The autotuner (coordinate search over launch options on a calibration slice of the
         # same B-scan: the first 1/div of every A-scan's photons), before the timed region;
@@ -269,13 +277,14 @@ def main():
         o, rate = ctx.autotune(mid, params, scans, seed, calib=calib)
         opts.block_size, opts.refill_quantum, opts.scatter_quantum = o.block_size, o.refill_quantum, o.scatter_quantum
         opts.tally_copies, opts.entry_skip, opts.record_fetch = o.tally_copies, o.entry_skip, o.record_fetch
+        opts.blocks_per_cu = o.blocks_per_cu
         tuned = {"block_size": o.block_size, "refill_quantum": o.refill_quantum,
                  "scatter_quantum": o.scatter_quantum, "tally_copies": o.tally_copies, "entry_skip": o.entry_skip,
-                 "record_fetch": o.record_fetch, "calibration_photons": calib,
+                 "record_fetch": o.record_fetch, "blocks_per_cu": o.blocks_per_cu, "calibration_photons": calib,
                  "seconds": time.perf_counter() - ts}
-        log(f"autotune ({tuned['seconds']:.1f} s): block {o.block_size} scatter/refill quanta "
-            f"{o.scatter_quantum}/{o.refill_quantum} tally copies {o.tally_copies} entry skip {o.entry_skip} "
-            f"record fetch {o.record_fetch} ({rate:.3e} photons/s on the calibration slice)")
+        log(f"autotune ({tuned['seconds']:.1f} s): block {o.block_size} blocks/CU {o.blocks_per_cu or 'max'} "
+            f"scatter/refill quanta {o.scatter_quantum}/{o.refill_quantum} tally copies {o.tally_copies} "
+            f"entry skip {o.entry_skip} record fetch {o.record_fetch} ({rate:.3e} photons/s on the calibration slice)")
     # Warmup: the logged kernel variant once (a different kernel symbol, so a profiler's
     # per-kernel average of the unlogged single-set transport_kernel covers the timed launches
     # only), then W steps of the unlogged graph launch on a 1/div slice of the B-scan.
@@ -395,7 +404,12 @@ def main():
         bscans = bscans[::a.bump_stride]
         bmid = ctx.upload(bm)
         bo = O.BatchOpts()
-        bo.use_graph, bo.scatter_quantum, bo.refill_quantum, bo.record_fetch = 1, 32, 16, 2
+        # The autotuner's choice for this mesh (octmps_autotune on a 4e6-photon slice of the
+        # sample): 4 blocks per CU, fewer photons in flight than the occupancy maximum, since
+        # the records do not fit the caches; the profiled options.
+        bq = [int(v) for v in a.bump_quanta.split(",")]
+        bo.use_graph, bo.record_fetch = 1, 2
+        bo.scatter_quantum, bo.refill_quantum, bo.blocks_per_cu = bq[0], bq[1], bq[2]
         _, bst = ctx.run_bscan(bmid, bpar, bscans, bseed, opts=bo)
         ctx.release(bmid)
         del bm
@@ -403,7 +417,8 @@ def main():
         b_alg = bsteps * ALG_BYTES_PER_STEP / (bms * 1e-3) / 1e9
         b_hbm = bsteps * PROFILE["bump_hbm_bytes_per_step"] / (bms * 1e-3) / 1e9
         bump = {"workload": f"bump: 6x6x3 mm, 128x128x64 cells x 6 tets (6,291,456 tets), {len(bscans)} of 1024 "
-                            f"A-scans x 1e6 photons, 1024 bins to 2 mm, quanta 32/16, cooperative record fetch",
+                            f"A-scans x 1e6 photons, 1024 bins to 2 mm, quanta {bq[0]}/{bq[1]}, {bq[2]} blocks per "
+                            f"CU, cooperative record fetch",
                 "photons": bst.photons, "kernel_ms": bms, "photons_per_s": bst.photons / (bms * 1e-3),
                 "photon_steps": bsteps, "steps_per_photon": bsteps / bst.photons,
                 "roofline": {"bound": "hbm", "achieved": b_alg, "peak": HBM_PEAK_GBS, "unit": "GB/s",

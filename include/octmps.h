@@ -319,13 +319,15 @@ octmps_status octmps_run_bscan_logged(octmps_ctx *ctx, int32_t mesh_id, const oc
                                       const octmps_batch_opts *opts, uint64_t *tallies_out,
                                       octmps_run_stats *stats, void *photon_log,
                                       uint64_t log_records);
-/* Autotune scatter quantum, block size, refill quantum, tally privatisation depth and the
- * entry-plane skip on a calibration slice of a B-scan (coordinate search from the context
- * defaults, 19 launches: a candidate must be more than 1 % faster to win, and the winner must
- * lead the starting point again on a second measurement of both);
- * writes the winner into *opts (scatter_quantum, block_size, refill_quantum, tally_copies,
- * entry_skip, record_fetch), makes the first four the context's default (entry_skip and
- * record_fetch stay per launch, since their best values depend on the mesh), and returns its
+/* Autotune occupancy (blocks per CU), scatter quantum, block size, refill quantum, tally
+ * privatisation depth and the record path on a calibration slice of a B-scan (coordinate
+ * search from the context defaults, 21 launches: a candidate must be more than 1 % faster to
+ * win, and the winner must lead the starting point again on a second measurement of both);
+ * writes the winner into *opts (blocks_per_cu, scatter_quantum, block_size, refill_quantum,
+ * tally_copies, entry_skip, record_fetch), makes block size, quanta and tally copies the
+ * context's default (occupancy and the record path stay per launch, since their best values
+ * depend on the mesh: meshes larger than the caches run faster with fewer photons in
+ * flight), and returns its
  * photons/s in *rate. The slice is the first calib_photons / n_scans photons of every A-scan,
  * raised to at least 8 photons per lane of a full-occupancy grid (about 3.1e6 on 256 CUs)
  * where the B-scan holds that many: a smaller slice is mostly the launch's tail. */

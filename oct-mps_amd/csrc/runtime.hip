@@ -488,7 +488,7 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     // (per-lane gathers with or without the entry-plane skip). Launch options move a calibration launch by a few percent at most, so a candidate
     // replaces the incumbent only when it is more than 1 % faster, and the final winner must
     // beat the starting point again on a second measurement of both: the noise of a single
-    // launch no longer picks the options (19 calibration launches).
+    // launch no longer picks the options (21 calibration launches).
     constexpr double kMargin = 1.01;
     octmps_batch_opts start = *opts;
     start.block_size = default_block(c);
@@ -497,6 +497,7 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     start.tally_copies = default_copies(c);
     start.entry_skip = 0;
     start.record_fetch = 0;
+    start.blocks_per_cu = 0;
     start.use_graph = 0;
     auto measure = [&](const octmps_batch_opts &o, double &r) -> octmps_status {
         octmps_run_stats st;
@@ -517,6 +518,14 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
         }
         return st;
     };
+    // Occupancy first: fewer resident blocks per CU (fewer photons in flight, a smaller working
+    // set) pays on meshes whose records do not fit the caches, while cache-resident meshes want
+    // every wave; the quanta are then tuned at the chosen occupancy.
+    for (int bpc : {4, 5}) {
+        octmps_batch_opts o = win;
+        o.blocks_per_cu = bpc;
+        if ((s = trial(o)) != OCTMPS_OK) return s;
+    }
     for (int sq : {20, 24, 32, 40}) {
         octmps_batch_opts o = win;
         o.scatter_quantum = sq;
@@ -535,6 +544,8 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     for (int b : {128, 192}) {
         octmps_batch_opts o = win;
         o.block_size = b;
+        // The same lanes per CU when the occupancy is limited.
+        if (win.blocks_per_cu > 0) o.blocks_per_cu = std::max(1, win.blocks_per_cu * win.block_size / b);
         if (b != win.block_size && (s = trial(o)) != OCTMPS_OK) return s;
     }
     for (int k : {2, 4}) {
@@ -568,6 +579,7 @@ extern "C" octmps_status octmps_autotune(octmps_ctx *c, int32_t mesh_id, const o
     opts->tally_copies = win.tally_copies;
     opts->entry_skip = win.entry_skip;
     opts->record_fetch = win.record_fetch;
+    opts->blocks_per_cu = win.blocks_per_cu;
     c->tuned_block = win.block_size;
     c->tuned_q = win.refill_quantum;
     c->tuned_sq = win.scatter_quantum;
