@@ -409,12 +409,16 @@ void scheduler(octmps_server *s) {
         // The open A-scans' first unclaimed photon and photons left are read under the lock
         // (a cancel lowers `limit` from another thread).
         bool sets_in_lds = true;
+        // The cooperative record fetch when every set in flight also fits its smaller LDS
+        // table (regions < kCoopSetRegions) and the blocks fit its stage (at most 4 waves).
+        bool coop = s->block <= 256;
         int n_lds = 0;
         std::vector<OpenScan> os;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (Req *r : s->active) {
                 if (r->me->max_region >= kLdsSetRegions || n_lds == kLdsSets) sets_in_lds = false;
+                if (r->me->max_region >= kCoopSetRegions) coop = false;
                 if (sets_in_lds) {
                     s->set_slot_h[r->set] = (uint8_t)n_lds;
                     s->lds_ids_h[n_lds++] = r->set;
@@ -500,6 +504,7 @@ void scheduler(octmps_server *s) {
         for (const OpenScan &o : os) unclaimed += o.rem;
         A.exit_on_drain = (total > 0 && total < unclaimed) ? 1u : 0u;
         A.sets_in_lds = sets_in_lds ? 1u : 0u;
+        A.coop_fetch = (sets_in_lds && coop) ? 1u : 0u;
         A.set_slot = (const uint8_t *)s->set_map.p;
         A.lds_set_ids = (const int32_t *)((const uint8_t *)s->set_map.p + sizeof s->set_slot_h);
         uint64_t ctrl[CTRL_WORDS] = {0};

@@ -13,6 +13,7 @@
 // needed a returning add to propagate its carry).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -264,7 +265,8 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int src) {
 // the crossing phase that needs it. Parked lanes read their exit face from the stage.
 template <bool MULTI, bool LOG, bool SKIP, bool SETS_LDS, bool COOP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) transport_kernel(KArgs A) {
-    static_assert(!(COOP && (MULTI || SKIP)), "the cooperative fetch is a single-set variant without the skip");
+    static_assert(!(COOP && SKIP), "the cooperative fetch does not skip the entry plane");
+    static_assert(!(COOP && MULTI && !SETS_LDS), "the multi-set cooperative variant keeps its sets in LDS");
     const uint32_t lane = __lane_id();
     const uint64_t gl = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 
@@ -292,7 +294,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     // its steps in layered tissue. MULTI with SETS_LDS keeps the first kLdsSets sets and
     // their region tables in LDS; MULTI without it reads each lane's set from memory.
     const DevSet S0 = A.sets[0];
-    constexpr int kRegionSlots = MULTI ? (SETS_LDS ? kLdsSets * kLdsSetRegions : 1) : (COOP ? 1 : kMaxRegions);
+    // Regions per set in the LDS set table (fewer beside the cooperative fetch's stage).
+    constexpr int kSetRegions = COOP ? kCoopSetRegions : kLdsSetRegions;
+    constexpr int kRegionSlots = MULTI ? (SETS_LDS ? kLdsSets * kSetRegions : 1) : (COOP ? 1 : kMaxRegions);
     __shared__ DevRegion lds_regions[kRegionSlots];
     __shared__ DevSet lds_sets[MULTI && SETS_LDS ? kLdsSets : 1];
     // COOP: the stages of the (at most four) waves in their own LDS object, so that the
@@ -308,8 +312,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             for (int i = threadIdx.x; i < kMaxRegions; i += blockDim.x) lds_regions[i] = A.regions[S0.region_base + i];
         __syncthreads();
     } else if (SETS_LDS) {
-        for (int i = threadIdx.x; i < kLdsSets * kLdsSetRegions; i += blockDim.x)
-            lds_regions[i] = A.regions[A.lds_set_ids[i / kLdsSetRegions] * kMaxRegions + i % kLdsSetRegions];
+        for (int i = threadIdx.x; i < kLdsSets * kSetRegions; i += blockDim.x)
+            lds_regions[i] = A.regions[A.lds_set_ids[i / kSetRegions] * kMaxRegions + i % kSetRegions];
         for (int i = threadIdx.x; i < kLdsSets; i += blockDim.x) lds_sets[i] = A.sets[A.lds_set_ids[i]];
         __syncthreads();
     }
@@ -317,10 +321,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     // A-scan, written at launch, and the exit face of a lane parked at an interface (normal
     // and packed neighbour word), written when it parks, so that the parked phase reads them
     // from LDS instead of making a dependent global round trip.
-    // COOP reads the face from the lane's stage and keeps the probe in dynamic LDS.
-    __shared__ float4 lds_probe_s[COOP ? 1 : 256], lds_face[COOP ? 1 : 256];
+    // COOP reads the face from the lane's stage; the single-set COOP variant keeps the probe
+    // in dynamic LDS.
+    constexpr bool kProbeDyn = COOP && !MULTI;
+    __shared__ float4 lds_probe_s[kProbeDyn ? 1 : 256], lds_face[COOP ? 1 : 256];
     auto lds_probe_at = [&]() -> float4 & {
-        return COOP ? lds_dyn[threadIdx.x] : lds_probe_s[threadIdx.x];
+        return kProbeDyn ? lds_dyn[threadIdx.x] : lds_probe_s[threadIdx.x];
     };
     // This lane's four planes in the stage: owner l was fetched by instruction l & 3, quad
     // l >> 2, at 65 (l & 3) + 4 (l >> 2).
@@ -328,7 +334,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     const uint4 *const own_links = reinterpret_cast<const uint4 *>(lds_stage + (threadIdx.x >> 6) * kStageF4 + kStageLinks + lane);
     auto region_rec = [&](int r) -> const DevRegion & {
         // The region table of set s starts at s * kMaxRegions (DevSet::region_base).
-        return MULTI ? (SETS_LDS ? lds_regions[set * kLdsSetRegions + r] : A.regions[set * kMaxRegions + r])
+        return MULTI ? (SETS_LDS ? lds_regions[set * kSetRegions + r] : A.regions[set * kMaxRegions + r])
                      : (COOP ? coop_regions[r] : lds_regions[r]);
     };
     auto load_region = [&](int r) {
@@ -354,7 +360,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             load_region(region);
             const DevScan &sc = A.slots[slot];
             lds_probe_at() = make_float4(sc.px, sc.py, sc.pz, 0.0f);
-            if (flags & F_IFACE) {
+            if (COOP) {
+                // The stage of this launch holds nothing yet: every restored photon is fetched
+                // below, before the loop (a parked one reads its exit face from the stage).
+                flags &= ~F_STAGED;
+            } else if (flags & F_IFACE) {
                 const uint32_t f = (flags & F_FACE) >> kFaceShift;
                 const float4 pl = A.tets[tet].plane[f];
                 lds_face[threadIdx.x] = make_float4(pl.x, pl.y, pl.z, __uint_as_float(A.tets[tet].link[f]));
@@ -441,6 +451,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
 #pragma clang diagnostic pop
         if (want) flags |= F_STAGED;
     };
+
+    if (COOP && SLICES && A.restore) {
+        // Restored photons, parked ones included, get their tets staged, and the stage is
+        // complete before the loop reads it.
+        coop_fetch((flags & F_ALIVE) != 0);
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        asm volatile("" ::: "memory");
+    }
 
     for (;;) {
         // ---- refill dead lanes (wave-uniform decision)
@@ -974,6 +992,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
         if (COOP) coop_fetch((flags & (F_ALIVE | F_PEND | F_IFACE | F_STAGED)) == F_ALIVE);
     }
 
+    // A slice can end with stage loads of the last iteration in flight: they complete into
+    // this block's LDS before the wave ends.
+    if (COOP) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        asm volatile("" ::: "memory");
+    }
     if (SLICES && A.slice_mode) {
         SavedLane &s = A.save[gl];
         s.flags = flags;
@@ -1059,19 +1083,26 @@ void launch_single(const KArgs &a, int grid, int block, hipStream_t stream) {
 }
 
 // Multi-set variant with the parameter sets in LDS when the launch allows it.
+// Multi-set variant with the parameter sets in LDS when the launch allows it, and with the
+// cooperative record fetch when the sets' regions also fit its smaller LDS table.
 template <bool LOG>
 void launch_multi(const KArgs &a, int grid, int block, hipStream_t stream) {
-    if (a.sets_in_lds) hipLaunchKernelGGL((transport_kernel<true, LOG, false, true>), dim3(grid), dim3(block), 0, stream, a);
+    if (a.sets_in_lds && a.coop_fetch)
+        hipLaunchKernelGGL((transport_kernel<true, LOG, false, true, true>), dim3(grid), dim3(block), 0, stream, a);
+    else if (a.sets_in_lds) hipLaunchKernelGGL((transport_kernel<true, LOG, false, true>), dim3(grid), dim3(block), 0, stream, a);
     else hipLaunchKernelGGL((transport_kernel<true, LOG, false, false>), dim3(grid), dim3(block), 0, stream, a);
 }
 
+// Blocks per CU of the tightest multi-set variant: a server's slices may switch variants, and
+// its saved lanes need the same grid in every slice.
 template <bool LOG>
 int occupancy_multi(int block) {
-    int a = 0, b = 0;
+    int a = 0, b = 0, c = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, transport_kernel<true, LOG, false, true>, block, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, transport_kernel<true, LOG, false, false>, block, 0) != hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, transport_kernel<true, LOG, false, false>, block, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, transport_kernel<true, LOG, false, true, true>, block, 0) != hipSuccess)
         return 0;
-    return a < b ? a : b;
+    return std::min(a, std::min(b, c));
 }
 
 // Blocks per CU of the tighter of the two single-set variants, or of the cooperative one

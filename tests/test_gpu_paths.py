@@ -158,6 +158,38 @@ def test_served_sets_beyond_the_lds_cache_equal_batched(ctx, small_meshes):
         srv.close()
 
 
+def test_served_slices_switch_record_paths_and_equal_batched(ctx, small_meshes):
+    # A slice fetches records cooperatively when every set in flight has regions below 8;
+    # a 10-region mesh in flight sends the slice to the per-lane gathers (sets still in LDS).
+    # Slices switch between the two as requests end, and the photons saved by one slice
+    # resume in the other (a restored photon's tet is fetched again): both give the batched
+    # bits.
+    from test_gpu_core import _layered_grid
+
+    xyz, tets = _layered_grid(n=(8, 8, 10))
+    mesh10 = O.Mesh.build(xyz, tets)
+    mid10 = ctx.upload(mesh10)
+    regs = [configs._reg(1.33 + 0.01 * (i % 5), 0.05, 8.0, 0.9) for i in range(10)]
+    p10 = O.make_params(regs, num_bins=128, depth=0.06, **configs.BIAS)
+    root, z = mesh10.find_root(0.0, 0.0, 0.0)
+    s10 = [O.ascan(0.0, 0.0, z, root, 3000), O.ascan(0.02, 0.0, z, mesh10.find_root(0.02, 0.0, 0.0)[0], 2000, 1)]
+    ref10 = ctx.run_bscan(mid10, p10, s10, 9)[0]
+    refs = {k: ctx.run_bscan(v[1], v[2], v[3], v[4])[0] for k, v in small_meshes.items()}
+    srv = O.Server(ctx, slice_photons=2048)
+    try:
+        ids = {k: srv.submit(v[1], v[2], v[3], v[4]) for k, v in small_meshes.items()}
+        r10 = srv.submit(mid10, p10, s10, 9)
+        info, out = srv.wait(r10, 120)
+        assert info.status == 0 and np.array_equal(out, ref10)
+        for k, rid in ids.items():
+            info, out = srv.wait(rid, 120)
+            assert info.status == 0 and np.array_equal(out, refs[k]), k
+        assert srv.stats()["slices"] > 2
+    finally:
+        srv.close()
+        ctx.release(mid10)
+
+
 def test_hand_out_policies_give_the_same_bits_and_order_service(ctx, small_meshes):
     # A large request (4 A-scans x 4e6 photons, ~3 s) and a small one (1 A-scan x 1e5)
     # submitted together. Least-remaining-first hands out the small request's photons first,
