@@ -366,7 +366,7 @@ def main():
         # The same B-scan as one request to the request server (the multi-set kernel in
         # wall-clock slices): its rate beside the batched launch and its tallies against the
         # batched launch's, bit for bit (the batched tallies come from the timed steps below).
-        srv = O.Server(ctx)
+        srv = O.Server(ctx, record_fetch=1 if a.record_fetch == 1 else 0)
         try:
             ts = time.perf_counter()
             rid = srv.submit(mid, params, scans, seed)
@@ -376,7 +376,7 @@ def main():
         finally:
             srv.close()
         served_same = {"photons_per_s": total_photons / sw, "wall_s": sw, "slices": sst["slices"],
-                       "status": info.status}
+                       "status": info.status, "record_fetch": "per-lane" if a.record_fetch == 1 else "auto"}
         log(f"B-scan as one served request: {total_photons / sw:.3e} photons/s")
 
     # Ceiling of the vector-memory path for divergent record gathers (cache-resident record

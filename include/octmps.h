@@ -355,7 +355,14 @@ typedef struct {
                                     0 = least remaining photons first, aged by waiting time
                                     (halved after 10 s, a third after 20 s, ...);
                                     1 = equal shares per A-scan. Results are the same bits. */
-    int32_t reserved;            /* must be 0 */
+    int32_t record_fetch;        /* record path of the slices: 0 = auto (cooperative, with the
+                                    parameter sets in flight in its LDS table when they fit
+                                    it: 16 sets of up to 8 regions or 8 of up to 16, read
+                                    from memory when they fit no LDS table; per-lane gathers
+                                    with the sets in LDS when they fit only the per-lane
+                                    table of 16 sets of 16 regions), 1 = per-lane gathers
+                                    always, 2 = the same as 0.
+                                    Results are the same bits. */
 } octmps_serve_opts;
 
 typedef struct {

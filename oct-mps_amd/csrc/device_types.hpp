@@ -24,9 +24,10 @@ constexpr int kMaxRegions = 100;
 // Parameter sets (and regions of each) a multi-set launch can keep in LDS, and the number
 // of parameter set ids a request server hands out.
 constexpr int kLdsSets = 16, kLdsSetRegions = 16, kMaxServeSets = 256;
-// Regions per parameter set the cooperative-fetch serving variant keeps in LDS (its record
-// stage leaves room for fewer): a slice uses that variant when every set in flight has fewer.
-constexpr int kCoopSetRegions = 8;
+// Region slots of the cooperative-fetch serving variant's LDS set table (its record stage
+// leaves room for fewer than kLdsSets * kLdsSetRegions): 16 sets of up to 8 regions, or 8
+// sets of up to 16, chosen per slice.
+constexpr int kCoopRegionSlots = 128;
 constexpr uint32_t kLinkBits = 25;                       // pool tet id bits of a packed link
 constexpr uint32_t kLinkMask = (1u << kLinkBits) - 1u;   // also the boundary id
 constexpr int64_t kMaxPoolTets = (int64_t)kLinkMask;     // ids 0 .. 2^25 - 2
@@ -143,10 +144,13 @@ struct KArgs {
     const TetRec *tets;
     uint32_t tets_bytes;        // bytes of resident records (<= (2^25 - 1) * 128 < 2^32)
     uint32_t skip_entry;        // 1: the plane of a tet's entry face is not fetched
-    uint32_t coop_fetch;        // 1: tet records fetched cooperatively into LDS (single-set, or
-                                // multi-set with the sets in LDS, regions < kCoopSetRegions)
+    uint32_t coop_fetch;        // 1: tet records fetched cooperatively into LDS (single-set;
+                                // multi-set with the sets in LDS or in global memory)
     uint32_t n_regions;         // coop_fetch: regions of the set (its LDS region table)
     uint32_t sets_in_lds;       // multi-set launch: every lane's set is in the LDS map, regions < kLdsSetRegions
+    uint32_t set_region_shift;  // sets_in_lds with coop_fetch: log2 of the region slots per set in
+                                // the cooperative variant's LDS table of kCoopRegionSlots regions
+                                // (3: 16 sets of up to 8 regions, 4: 8 sets of up to 16)
     const uint8_t *set_slot;    // sets_in_lds: LDS slot of each parameter set id [kMaxServeSets]
     const int32_t *lds_set_ids; // sets_in_lds: parameter set id held in each LDS slot [kLdsSets]
     const DevRegion *regions;

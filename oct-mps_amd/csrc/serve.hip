@@ -409,16 +409,13 @@ void scheduler(octmps_server *s) {
         // The open A-scans' first unclaimed photon and photons left are read under the lock
         // (a cancel lowers `limit` from another thread).
         bool sets_in_lds = true;
-        // The cooperative record fetch when every set in flight also fits its smaller LDS
-        // table (regions < kCoopSetRegions) and the blocks fit its stage (at most 4 waves).
-        bool coop = s->block <= 256;
-        int n_lds = 0;
+        int n_lds = 0, region_slots = 1;  // region slots per set: the largest region index + 1
         std::vector<OpenScan> os;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (Req *r : s->active) {
                 if (r->me->max_region >= kLdsSetRegions || n_lds == kLdsSets) sets_in_lds = false;
-                if (r->me->max_region >= kCoopSetRegions) coop = false;
+                region_slots = std::max(region_slots, r->me->max_region + 1);
                 if (sets_in_lds) {
                     s->set_slot_h[r->set] = (uint8_t)n_lds;
                     s->lds_ids_h[n_lds++] = r->set;
@@ -448,6 +445,20 @@ void scheduler(octmps_server *s) {
             seg_slot.push_back(0);
             seg_poff.push_back(0);
             seg_start.insert(seg_start.begin(), 0);
+        }
+        // Kernel variant of the slice. The cooperative record fetch runs whenever the blocks
+        // fit its stage (at most 4 waves) and the options allow it. Beside its stage the LDS
+        // set table has kCoopRegionSlots slots: 16 sets of up to 8 regions or 8 sets of up to
+        // 16. Sets that fit the per-lane variant's larger table (16 sets of 16) but not this
+        // one run the per-lane variant with the sets in LDS (region reads from global memory
+        // cost more there than the cooperative fetch saves); sets that fit no LDS table run
+        // the cooperative variant reading them from global memory.
+        bool coop = s->block <= 256 && s->o.record_fetch != 1;
+        uint32_t set_shift = 4;
+        if (coop && sets_in_lds) {
+            if (region_slots <= 8 && n_lds <= 16) set_shift = 3;
+            else if (region_slots <= 16 && n_lds <= 8) set_shift = 4;
+            else coop = false;
         }
         for (int j = n_lds; j < kLdsSets; ++j) s->lds_ids_h[j] = s->lds_ids_h[0];  // unused slots: any valid id
         hipError_t e;
@@ -504,7 +515,8 @@ void scheduler(octmps_server *s) {
         for (const OpenScan &o : os) unclaimed += o.rem;
         A.exit_on_drain = (total > 0 && total < unclaimed) ? 1u : 0u;
         A.sets_in_lds = sets_in_lds ? 1u : 0u;
-        A.coop_fetch = (sets_in_lds && coop) ? 1u : 0u;
+        A.coop_fetch = coop ? 1u : 0u;
+        A.set_region_shift = set_shift;
         A.set_slot = (const uint8_t *)s->set_map.p;
         A.lds_set_ids = (const int32_t *)((const uint8_t *)s->set_map.p + sizeof s->set_slot_h);
         uint64_t ctrl[CTRL_WORDS] = {0};
@@ -602,9 +614,9 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
         delete s;
         return fail(OCTMPS_E_INVALID, "policy must be 0 (least remaining first) or 1 (equal shares)");
     }
-    if (s->o.reserved != 0) {
+    if (s->o.record_fetch < 0 || s->o.record_fetch > 2) {
         delete s;
-        return fail(OCTMPS_E_INVALID, "reserved option must be 0");
+        return fail(OCTMPS_E_INVALID, "record_fetch must be 0 (auto), 1 (per-lane) or 2 (cooperative where it fits)");
     }
     if (hipError_t se = hipSetDevice(ctx->device); se != hipSuccess) {
         delete s;

@@ -264,9 +264,8 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int src) {
 // LDS, and a photon that moved fetches its new tet at the end of the iteration, ahead of
 // the crossing phase that needs it. Parked lanes read their exit face from the stage.
 template <bool MULTI, bool LOG, bool SKIP, bool SETS_LDS, bool COOP = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) transport_kernel(KArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 5 : 6))) transport_kernel(KArgs A) {
     static_assert(!(COOP && SKIP), "the cooperative fetch does not skip the entry plane");
-    static_assert(!(COOP && MULTI && !SETS_LDS), "the multi-set cooperative variant keeps its sets in LDS");
     const uint32_t lane = __lane_id();
     const uint64_t gl = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 
@@ -294,9 +293,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     // its steps in layered tissue. MULTI with SETS_LDS keeps the first kLdsSets sets and
     // their region tables in LDS; MULTI without it reads each lane's set from memory.
     const DevSet S0 = A.sets[0];
-    // Regions per set in the LDS set table (fewer beside the cooperative fetch's stage).
-    constexpr int kSetRegions = COOP ? kCoopSetRegions : kLdsSetRegions;
-    constexpr int kRegionSlots = MULTI ? (SETS_LDS ? kLdsSets * kSetRegions : 1) : (COOP ? 1 : kMaxRegions);
+    // The LDS set table: set slot s holds its regions at s << set_shift (fewer slots beside
+    // the cooperative fetch's stage, split per launch between sets and regions per set).
+    constexpr int kRegionSlots =
+        MULTI ? (SETS_LDS ? (COOP ? kCoopRegionSlots : kLdsSets * kLdsSetRegions) : 1) : (COOP ? 1 : kMaxRegions);
+    const uint32_t set_shift = COOP ? A.set_region_shift : 4u;
+    static_assert(kLdsSetRegions == 16, "per-lane set table: 16 regions per set (shift 4)");
     __shared__ DevRegion lds_regions[kRegionSlots];
     __shared__ DevSet lds_sets[MULTI && SETS_LDS ? kLdsSets : 1];
     // COOP: the stages of the (at most four) waves in their own LDS object, so that the
@@ -312,8 +314,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
             for (int i = threadIdx.x; i < kMaxRegions; i += blockDim.x) lds_regions[i] = A.regions[S0.region_base + i];
         __syncthreads();
     } else if (SETS_LDS) {
-        for (int i = threadIdx.x; i < kLdsSets * kSetRegions; i += blockDim.x)
-            lds_regions[i] = A.regions[A.lds_set_ids[i / kSetRegions] * kMaxRegions + i % kSetRegions];
+        for (int i = threadIdx.x; i < kRegionSlots; i += blockDim.x)
+            lds_regions[i] = A.regions[A.lds_set_ids[i >> set_shift] * kMaxRegions + (i & ((1 << set_shift) - 1))];
         for (int i = threadIdx.x; i < kLdsSets; i += blockDim.x) lds_sets[i] = A.sets[A.lds_set_ids[i]];
         __syncthreads();
     }
@@ -334,7 +336,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) t
     const uint4 *const own_links = reinterpret_cast<const uint4 *>(lds_stage + (threadIdx.x >> 6) * kStageF4 + kStageLinks + lane);
     auto region_rec = [&](int r) -> const DevRegion & {
         // The region table of set s starts at s * kMaxRegions (DevSet::region_base).
-        return MULTI ? (SETS_LDS ? lds_regions[set * kSetRegions + r] : A.regions[set * kMaxRegions + r])
+        return MULTI ? (SETS_LDS ? lds_regions[(set << set_shift) + r] : A.regions[set * kMaxRegions + r])
                      : (COOP ? coop_regions[r] : lds_regions[r]);
     };
     auto load_region = [&](int r) {
@@ -1082,13 +1084,15 @@ void launch_single(const KArgs &a, int grid, int block, hipStream_t stream) {
     else hipLaunchKernelGGL((transport_kernel<false, LOG, false, false>), dim3(grid), dim3(block), 0, stream, a);
 }
 
-// Multi-set variant with the parameter sets in LDS when the launch allows it.
 // Multi-set variant with the parameter sets in LDS when the launch allows it, and with the
-// cooperative record fetch when the sets' regions also fit its smaller LDS table.
+// cooperative record fetch when the launch asks for it (sets in LDS: their regions fit the
+// smaller table beside the stage).
 template <bool LOG>
 void launch_multi(const KArgs &a, int grid, int block, hipStream_t stream) {
     if (a.sets_in_lds && a.coop_fetch)
         hipLaunchKernelGGL((transport_kernel<true, LOG, false, true, true>), dim3(grid), dim3(block), 0, stream, a);
+    else if (a.coop_fetch)
+        hipLaunchKernelGGL((transport_kernel<true, LOG, false, false, true>), dim3(grid), dim3(block), 0, stream, a);
     else if (a.sets_in_lds) hipLaunchKernelGGL((transport_kernel<true, LOG, false, true>), dim3(grid), dim3(block), 0, stream, a);
     else hipLaunchKernelGGL((transport_kernel<true, LOG, false, false>), dim3(grid), dim3(block), 0, stream, a);
 }
@@ -1097,12 +1101,13 @@ void launch_multi(const KArgs &a, int grid, int block, hipStream_t stream) {
 // its saved lanes need the same grid in every slice.
 template <bool LOG>
 int occupancy_multi(int block) {
-    int a = 0, b = 0, c = 0;
+    int a = 0, b = 0, c = 0, d = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, transport_kernel<true, LOG, false, true>, block, 0) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, transport_kernel<true, LOG, false, false>, block, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, transport_kernel<true, LOG, false, true, true>, block, 0) != hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, transport_kernel<true, LOG, false, true, true>, block, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, transport_kernel<true, LOG, false, false, true>, block, 0) != hipSuccess)
         return 0;
-    return std::min(a, std::min(b, c));
+    return std::min(std::min(a, b), std::min(c, d));
 }
 
 // Blocks per CU of the tighter of the two single-set variants, or of the cooperative one

@@ -107,7 +107,7 @@ class ServeOpts(C.Structure):
     _fields_ = [
         ("max_inflight_scans", C.c_int32), ("block_size", C.c_int32), ("refill_quantum", C.c_int32),
         ("fail_request_index", C.c_int32), ("hbm_budget_bytes", C.c_uint64), ("slice_photons", C.c_uint64),
-        ("policy", C.c_int32), ("reserved", C.c_int32),
+        ("policy", C.c_int32), ("record_fetch", C.c_int32),
     ]
 
 

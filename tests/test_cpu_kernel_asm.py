@@ -15,9 +15,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "oct-mps_amd", "csrc")
 HIPCC = "/opt/rocm/bin/hipcc"
 # transport_kernel<MULTI, LOG, SKIP, SETS_LDS, COOP> with COOP = true: single-set and the
-# serving (multi-set, sets in LDS) variant, unlogged and logged.
+# serving (multi-set, sets in LDS or in global memory) variants, unlogged and logged.
 COOP_KERNELS = ("ILb0ELb0ELb0ELb0ELb1EEEvNS_5KArgsE", "ILb0ELb1ELb0ELb0ELb1EEEvNS_5KArgsE",
-                "ILb1ELb0ELb0ELb1ELb1EEEvNS_5KArgsE", "ILb1ELb1ELb0ELb1ELb1EEEvNS_5KArgsE")
+                "ILb1ELb0ELb0ELb1ELb1EEEvNS_5KArgsE", "ILb1ELb1ELb0ELb1ELb1EEEvNS_5KArgsE",
+                "ILb1ELb0ELb0ELb0ELb1EEEvNS_5KArgsE", "ILb1ELb1ELb0ELb0ELb1EEEvNS_5KArgsE")
 
 
 @pytest.fixture(scope="module")
@@ -59,10 +60,11 @@ def test_m0_is_written_only_by_the_fetch_blocks(asm, suffix):
         assert sum(l.startswith(("s_mov_b32 m0", "s_add_u32 m0")) for l in lines) == 5
 
 
-@pytest.mark.parametrize("suffix", (COOP_KERNELS[0], COOP_KERNELS[2]))
+@pytest.mark.parametrize("suffix", (COOP_KERNELS[0], COOP_KERNELS[2], COOP_KERNELS[4]))
 def test_cooperative_kernel_does_not_spill(asm, suffix):
     tail_start = asm.index("_ZN6octmps12_GLOBAL__N_116transport_kernel" + suffix + ":")
     tail = asm[tail_start:asm.index("; Occupancy", tail_start) + 200]
     assert re.search(r"; ScratchSize: 0\b", tail), "the unlogged cooperative kernel spills registers"
-    # 6 waves per SIMD single-set; the serving variant's LDS (stage + set table) allows 5.
+    # 6 waves per SIMD single-set; the serving variants are built for 5, the serving grid's
+    # occupancy (the LDS of the variant with the set table allows 5 blocks per CU).
     assert re.search(r"; Occupancy: %d\b" % (6 if suffix == COOP_KERNELS[0] else 5), tail)

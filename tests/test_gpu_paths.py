@@ -117,12 +117,14 @@ def test_served_results_equal_batched_alone_and_in_a_mix(ctx, small_meshes):
         srv.close()
 
 
-def test_served_sets_beyond_the_lds_cache_equal_batched(ctx, small_meshes):
-    # A slice keeps the parameter sets in LDS only while every request in flight has a set
-    # id below 16 and a mesh whose regions are below 16; otherwise the kernel reads each
-    # lane's set from memory. Both give the batched bits: 20 requests with 20 different
-    # parameter sets submitted at once (set ids up to 19, so slices switch between the two
-    # as requests end), then a 40-region mesh beside a small one.
+@pytest.mark.parametrize("record_fetch", [0, 1])
+def test_served_sets_beyond_the_lds_cache_equal_batched(ctx, small_meshes, record_fetch):
+    # A slice keeps the parameter sets in LDS only while at most 16 requests are in flight
+    # and their meshes' regions fit the LDS table; otherwise the kernel reads each lane's set
+    # from memory. Both give the batched bits, with the cooperative record fetch (0) and the
+    # per-lane gathers (1): 20 requests with 20 different parameter sets submitted at once
+    # (slices switch between the two as requests end), then a 40-region mesh beside a small
+    # one.
     from test_gpu_core import _layered_grid
 
     _, mid, p, s, seed = small_meshes["sphere"]
@@ -143,7 +145,7 @@ def test_served_sets_beyond_the_lds_cache_equal_batched(ctx, small_meshes):
     ref40 = ctx.run_bscan(mid40, p40, s40, 5)[0]
     _, mids, ps, ss, seeds = small_meshes["slab"]
     ref_slab = ctx.run_bscan(mids, ps, ss, seeds)[0]
-    srv = O.Server(ctx, slice_photons=4096)
+    srv = O.Server(ctx, slice_photons=4096, record_fetch=record_fetch)
     try:
         ids = [srv.submit(mid, q, s, seed + i) for i, q in enumerate(sets)]
         for i, rid in enumerate(ids):
@@ -158,12 +160,14 @@ def test_served_sets_beyond_the_lds_cache_equal_batched(ctx, small_meshes):
         srv.close()
 
 
-def test_served_slices_switch_record_paths_and_equal_batched(ctx, small_meshes):
-    # A slice fetches records cooperatively when every set in flight has regions below 8;
-    # a 10-region mesh in flight sends the slice to the per-lane gathers (sets still in LDS).
-    # Slices switch between the two as requests end, and the photons saved by one slice
-    # resume in the other (a restored photon's tet is fetched again): both give the batched
-    # bits.
+def test_served_slices_switch_kernel_variants_and_equal_batched(ctx, small_meshes):
+    # The LDS set table beside the cooperative fetch's stage holds 16 sets of up to 8
+    # regions or 8 sets of up to 16; sets that fit neither but fit the per-lane variant's
+    # table (16 of 16) run the per-lane gathers. Eleven requests in flight, one of them on an
+    # 11-region mesh, start on the per-lane variant; as requests end the slices move to the
+    # cooperative 8 x 16 table and, once the 11-region request is done, to 16 x 8 (a long
+    # request outlives it). Photons saved by one slice resume in the next whatever variant
+    # runs it (a restored photon's tet is fetched again): every request gives the batched bits.
     from test_gpu_core import _layered_grid
 
     xyz, tets = _layered_grid(n=(8, 8, 10))
@@ -174,10 +178,14 @@ def test_served_slices_switch_record_paths_and_equal_batched(ctx, small_meshes):
     root, z = mesh10.find_root(0.0, 0.0, 0.0)
     s10 = [O.ascan(0.0, 0.0, z, root, 3000), O.ascan(0.02, 0.0, z, mesh10.find_root(0.02, 0.0, 0.0)[0], 2000, 1)]
     ref10 = ctx.run_bscan(mid10, p10, s10, 9)[0]
-    refs = {k: ctx.run_bscan(v[1], v[2], v[3], v[4])[0] for k, v in small_meshes.items()}
+    reqs = {(k, j): (v[1], v[2], v[3], v[4] + j) for k, v in small_meshes.items() for j in range(3)}
+    _, mids, ps, ss, seeds = small_meshes["slab"]
+    reqs[("slab_long", 0)] = (mids, ps, [O.ascan(x.x, x.y, x.z, x.root_tet, 30000, x.stream_salt) for x in ss],
+                              seeds + 7)
+    refs = {k: ctx.run_bscan(*v)[0] for k, v in reqs.items()}
     srv = O.Server(ctx, slice_photons=2048)
     try:
-        ids = {k: srv.submit(v[1], v[2], v[3], v[4]) for k, v in small_meshes.items()}
+        ids = {k: srv.submit(*v) for k, v in reqs.items()}
         r10 = srv.submit(mid10, p10, s10, 9)
         info, out = srv.wait(r10, 120)
         assert info.status == 0 and np.array_equal(out, ref10)
@@ -188,6 +196,21 @@ def test_served_slices_switch_record_paths_and_equal_batched(ctx, small_meshes):
     finally:
         srv.close()
         ctx.release(mid10)
+
+
+def test_served_per_lane_record_path_equals_batched(ctx, small_meshes):
+    # record_fetch 1 keeps every slice on the per-lane gathers, even where the cooperative
+    # fetch fits; small slices make photons cross slice boundaries (save and restore).
+    refs = {k: ctx.run_bscan(v[1], v[2], v[3], v[4])[0] for k, v in small_meshes.items()}
+    srv = O.Server(ctx, slice_photons=4096, record_fetch=1)
+    try:
+        ids = {k: srv.submit(v[1], v[2], v[3], v[4]) for k, v in small_meshes.items()}
+        for k, rid in ids.items():
+            info, out = srv.wait(rid, 120)
+            assert info.status == 0 and np.array_equal(out, refs[k]), k
+        assert srv.stats()["slices"] > 2
+    finally:
+        srv.close()
 
 
 def test_hand_out_policies_give_the_same_bits_and_order_service(ctx, small_meshes):
@@ -219,7 +242,7 @@ def test_hand_out_policies_give_the_same_bits_and_order_service(ctx, small_meshe
     assert lat[0][0] < lat[1][0]
 
 
-@pytest.mark.parametrize("field,value", [("policy", 2), ("policy", -1), ("reserved", 1), ("block_size", 320)])
+@pytest.mark.parametrize("field,value", [("policy", 2), ("policy", -1), ("record_fetch", 3), ("record_fetch", -1), ("block_size", 320)])
 def test_invalid_serve_options_are_rejected(ctx, field, value):
     with pytest.raises(O.OctmpsError) as e:
         O.Server(ctx, **{field: value})
