@@ -127,6 +127,17 @@ struct octmps_server {
     uint64_t fixed_bytes = 0;       // the server's own device state, outside the budget
     std::map<int32_t, MeshUse> mesh_use;
     std::vector<int32_t> to_evict;  // meshes dropped from the budget, evicted before the next slice
+    // Completion: a finished request's tallies are copied to the caller on their own thread and
+    // stream while the next slice runs, and the request ends (its resources freed, waiters
+    // woken) once its copy is done. tally_mu keeps the tally pool from being replaced under a
+    // copy (lock order: mu, then tally_mu; the copy thread never holds tally_mu while taking mu).
+    std::thread copy_th;
+    std::mutex copy_mu, tally_mu;
+    std::condition_variable cv_copy;
+    std::deque<Req *> copy_q;
+    bool copy_stop = false;
+    int copies_out = 0;  // requests handed to the copy thread and not yet ended (under mu)
+    hipStream_t copy_stream = nullptr;
     // stats
     std::atomic<uint64_t> st_uploads{0}, st_evictions{0};
     uint64_t st_photons = 0, st_steps = 0, st_slices = 0;
@@ -169,6 +180,7 @@ void range_free(std::vector<Range> &free, uint64_t off, uint64_t n) {
 // Grows the device tally pool to hold at least `words` more (a new allocation; the live
 // tallies are copied over). Called between slices, from the scheduler thread.
 bool grow_tallies(octmps_server *s, uint64_t words) {
+    std::lock_guard<std::mutex> tk(s->tally_mu);  // no completion copy reads the old pool
     const uint64_t old = s->tally_cap_words;
     const uint64_t ncap = std::max<uint64_t>(old * 2, old + words);
     void *np = nullptr;
@@ -294,7 +306,7 @@ void admit(octmps_server *s, std::vector<Req *> &staged) {
         uint64_t off;
         if (!alloc_tally(s, r->words * need_slots, off)) {
             range_free(s->free_slots, slot_off, need_slots);
-            if (s->active.empty()) {  // the device cannot hold its tallies at all
+            if (s->active.empty() && s->copies_out == 0) {  // the device cannot hold its tallies at all
                 s->pending.pop_front();
                 end_unadmitted(s, r, OCTMPS_E_NOMEM);
                 continue;
@@ -329,8 +341,36 @@ void finish(octmps_server *s, Req *r, int status) {
     r->mesh_active = false;
     if (r->counted) r->me->refs--;
     r->counted = false;
-    s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+    auto it = std::find(s->active.begin(), s->active.end(), r);
+    if (it != s->active.end()) s->active.erase(it);  // a request being copied has left it
     s->cv_done.notify_all();
+}
+
+// Copies finished requests' tallies to their callers and ends them, in hand-off order, while
+// the scheduler runs the next slices. Leaves once the server closes and the queue is empty.
+void copier(octmps_server *s) {
+    (void)hipSetDevice(s->ctx->device);
+    for (;;) {
+        Req *r = nullptr;
+        {
+            std::unique_lock<std::mutex> ck(s->copy_mu);
+            s->cv_copy.wait(ck, [&] { return s->copy_stop || !s->copy_q.empty(); });
+            if (s->copy_q.empty()) return;
+            r = s->copy_q.front();
+            s->copy_q.pop_front();
+        }
+        hipError_t e;
+        {
+            std::lock_guard<std::mutex> tk(s->tally_mu);
+            e = hipMemcpyAsync(r->out, (uint64_t *)s->tallies.p + r->tally_base, r->words * r->scans.size() * 8,
+                               hipMemcpyDeviceToHost, s->copy_stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->copy_stream);
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        finish(s, r, e == hipSuccess ? OCTMPS_OK : OCTMPS_E_DEVICE);
+        s->copies_out--;
+        s->cv_work.notify_all();  // its slots, set and tallies are free for admission
+    }
 }
 
 // Before a slice, with the pool held exclusively: evicts the meshes admission dropped from
@@ -375,6 +415,13 @@ void scheduler(octmps_server *s) {
             while (!s->stop && s->pending.empty() && s->active.empty()) s->cv_work.wait(lk);
             if (s->stop && s->pending.empty() && s->active.empty()) break;
             admit(s, staged);
+            if (staged.empty() && s->active.empty() && s->copies_out > 0) {
+                // Nothing to run: the queued requests wait for resources that requests still
+                // being copied hold. Their copies wake the scheduler (meshes to evict stay
+                // listed for the next pass).
+                s->cv_work.wait(lk);
+                continue;
+            }
             evict.swap(s->to_evict);
         }
         if (!staged.empty() || !evict.empty()) {
@@ -582,11 +629,18 @@ void scheduler(octmps_server *s) {
                 if (st != OCTMPS_OK) finish(s, r, st);
                 continue;
             }
-            hipError_t ec = hipMemcpy(r->out, (uint64_t *)s->tallies.p + r->tally_base,
-                                      r->words * r->scans.size() * 8, hipMemcpyDeviceToHost);
+            // Complete: its tallies are final (none of its photons is left in a lane). The copy
+            // to the caller runs beside the next slice; the request keeps its slots, set and
+            // tally range until the copy is done.
             r->photons = ph;
             s->st_photons += ph;
-            finish(s, r, ec == hipSuccess ? OCTMPS_OK : OCTMPS_E_DEVICE);
+            s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+            s->copies_out++;
+            {
+                std::lock_guard<std::mutex> ck(s->copy_mu);
+                s->copy_q.push_back(r);
+            }
+            s->cv_copy.notify_one();
         }
     }
 }
@@ -637,6 +691,7 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
     s->tally_cap_words = (8ull << 20) / 8;
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&s->e0)) != hipSuccess || (e = hipEventCreate(&s->e1)) != hipSuccess ||
         (e = s->regions.ensure((size_t)kMaxSets * kMaxRegions * sizeof(DevRegion))) != hipSuccess ||
         (e = s->sets.ensure((size_t)kMaxSets * sizeof(DevSet))) != hipSuccess ||
@@ -656,6 +711,7 @@ extern "C" octmps_status octmps_serve_open(octmps_ctx *ctx, const octmps_serve_o
     s->free_slots.push_back({0, (uint64_t)s->n_slots});
     for (int i = kMaxSets - 1; i >= 0; --i) s->free_sets.push_back(i);
     s->free_tally.push_back({0, s->tally_cap_words});
+    s->copy_th = std::thread(copier, s);
     s->th = std::thread(scheduler, s);
     *out = s;
     return OCTMPS_OK;
@@ -814,6 +870,12 @@ extern "C" octmps_status octmps_serve_close(octmps_server *s) {
         s->cv_work.notify_all();
     }
     if (s->th.joinable()) s->th.join();
+    {
+        std::lock_guard<std::mutex> ck(s->copy_mu);
+        s->copy_stop = true;
+        s->cv_copy.notify_all();
+    }
+    if (s->copy_th.joinable()) s->copy_th.join();
     (void)hipSetDevice(s->ctx->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (DevBuf *b : {&s->regions, &s->sets, &s->slots, &s->seg_start, &s->seg_slot, &s->seg_poff, &s->tallies,
@@ -822,6 +884,7 @@ extern "C" octmps_status octmps_serve_close(octmps_server *s) {
     if (s->e0) (void)hipEventDestroy(s->e0);
     if (s->e1) (void)hipEventDestroy(s->e1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     for (auto &kv : s->reqs) delete kv.second;
     delete s;
     return OCTMPS_OK;

@@ -288,6 +288,44 @@ def test_serving_cancel_and_admission(ctx, small_meshes):
         srv.close()
 
 
+def test_completion_copies_hold_resources_until_done(ctx, small_meshes):
+    # A finished request's tallies are copied to the caller beside the next slice, and the
+    # request keeps its scan slots, parameter set and tally range until that copy is done.
+    # With slots for one request at a time, each of six requests is admitted only after the
+    # previous one's copy has ended, and every one gives the batched bits.
+    _, mid, p, s, seed = small_meshes["ellipsoid"]
+    refs = [ctx.run_bscan(mid, p, s, seed + k)[0] for k in range(6)]
+    srv = O.Server(ctx, max_inflight_scans=len(s), slice_photons=4096)
+    try:
+        ids = [srv.submit(mid, p, s, seed + k) for k in range(6)]
+        infos = []
+        for k, rid in enumerate(ids):
+            info, out = srv.wait(rid, 120)
+            assert info.status == 0 and np.array_equal(out, refs[k]), k
+            infos.append(info)
+        for a, b in zip(infos, infos[1:]):
+            assert b.t_admit_ms >= a.t_done_ms - 1e-6
+    finally:
+        srv.close()
+    # Requests arriving while others finish make the tally pool grow (it starts at 8 MiB;
+    # each of these holds 16 x 4096 bins, 7 MiB) beside the copies: the same bits.
+    mb, pb, sb, seedb = configs.slab4(cells=(16, 16, 8), n_scans=16, photons=3000, bins=4096)
+    midb = ctx.upload(mb)
+    refs = [ctx.run_bscan(midb, pb, sb, seedb + k)[0] for k in range(4)]
+    srv = O.Server(ctx, slice_photons=8192)
+    try:
+        ids = []
+        for k in range(4):
+            ids.append(srv.submit(midb, pb, sb, seedb + k))
+            time.sleep(0.02)
+        for k, rid in enumerate(ids):
+            info, out = srv.wait(rid, 120)
+            assert info.status == 0 and np.array_equal(out, refs[k]), k
+    finally:
+        srv.close()
+        ctx.release(midb)
+
+
 def test_translation_invariance_in_layers(ctx):
     # In a laterally uniform layered medium, A-scans away from the sides are statistically
     # the same (independent streams per A-scan).
