@@ -371,7 +371,9 @@ typedef struct {
     const octmps_params *params;   /* copied at submit */
     const octmps_ascan *scans;     /* copied at submit */
     uint64_t seed;
-    uint64_t *tallies_out;         /* n_scans * octmps_tally_words(B); written on completion */
+    uint64_t *tallies_out;         /* n_scans * octmps_tally_words(B); written by the server's
+                                      completion thread before the request is reported done,
+                                      so it must stay valid until then (or until close) */
 } octmps_request;
 
 typedef struct {
