@@ -619,6 +619,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
                 flags &= ~F_ENTRY;
                 __builtin_amdgcn_s_waitcnt(0x0F70);
                 asm volatile("" ::: "memory");
+                // From here until its next fetch is issued the wave has the stage's data, and
+                // runs ahead of waves that wait for theirs (issue priority 1, dropped to 0 after
+                // the end-of-iteration fetch): the next fetches go out sooner.
+                __builtin_amdgcn_s_setprio(1);
                 const uint4 L = *own_links;
                 const float4 p0 = own_planes[0], p1 = own_planes[1], p2 = own_planes[2], p3 = own_planes[3];
                 __builtin_amdgcn_sched_barrier(0);
@@ -991,7 +995,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
         // here, ahead of the next crossing phase. Fetching at the end of the crossing phase
         // instead made the parked phase wait for those loads (its reads of the stage, and of
         // spilled registers, wait for all of the wave's vector memory operations).
-        if (COOP) coop_fetch((flags & (F_ALIVE | F_PEND | F_IFACE | F_STAGED)) == F_ALIVE);
+        if (COOP) {
+            coop_fetch((flags & (F_ALIVE | F_PEND | F_IFACE | F_STAGED)) == F_ALIVE);
+            __builtin_amdgcn_s_setprio(0);
+        }
     }
 
     // A slice can end with stage loads of the last iteration in flight: they complete into
