@@ -33,9 +33,9 @@ LAYOUT_BYTES_PER_STEP = 80  # this build's packed tet record: 4 planes x 16 B + 
 # dependency) over SQ_WAVE_CYCLES.
 PROFILE = {
     # Cooperative record fetch, quanta 28/1.
-    "coop": {"l1_lookups_per_step": 1.72, "ta_busy": 0.64, "tcp_pending_l2": 0.39, "l2_hit": 0.717,
-             "l2_latency_cycles": 323, "wave_wait_dependency": 0.32, "wave_wait_issue": 0.34, "wave_issuing": 0.34,
-             "valu_per_step": 4.98, "salu_per_step": 1.74, "salu_per_cu_cycle": 0.464, "hbm_bytes_per_step": 24.98,
+    "coop": {"l1_lookups_per_step": 1.72, "ta_busy": 0.65, "tcp_pending_l2": 0.38, "l2_hit": 0.717,
+             "l2_latency_cycles": 324, "wave_wait_dependency": 0.27, "wave_wait_issue": 0.38, "wave_issuing": 0.35,
+             "valu_per_step": 4.98, "salu_per_step": 1.74, "salu_per_cu_cycle": 0.472, "hbm_bytes_per_step": 24.99,
              "source": "profiles/transport_slab4_memory.md, transport_slab4_sq.md, transport_slab4_pmc.md"},
     # Per-lane gathers, quanta 24/20, entry-plane skip on / off.
     "lane_skip_on": {"l1_lookups_per_step": 4.16, "ta_busy": 0.84, "tcp_pending_l2": 0.18, "valu_per_step": 5.59,
