@@ -13,9 +13,11 @@
 // policy (least remaining photons first with aging, or equal shares; serve_order.hpp);
 // during the slice, lanes whose photon dies pick up the next listed id; at the end
 // of a slice live photons are saved per lane and resumed by the next slice
-// (SaveThreadState/RestoreThreadState, src/octmps_kernel.cuh:1154-1327). Every photon's
-// streams depend only on (seed, salt, photon index), so a request's tallies are bitwise the
-// same alone or in any mix.
+// (SaveThreadState/RestoreThreadState, src/octmps_kernel.cuh:1154-1327). A request whose
+// photons are all done is handed to a completion thread, which copies its tallies to the
+// caller on its own stream while the next slice runs and then ends the request. Every
+// photon's streams depend only on (seed, salt, photon index), so a request's tallies are
+// bitwise the same alone or in any mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
