@@ -326,6 +326,20 @@ def test_completion_copies_hold_resources_until_done(ctx, small_meshes):
         ctx.release(midb)
 
 
+def test_close_finishes_queued_running_and_copying_requests(ctx, small_meshes):
+    # octmps_serve_close runs every submitted request to its end before it returns: requests
+    # still queued (slots for one request at a time), running, or having their tallies copied
+    # by the completion thread. Their output buffers then hold the batched bits.
+    _, mid, p, s, seed = small_meshes["ellipsoid"]
+    refs = [ctx.run_bscan(mid, p, s, seed + 20 + k)[0] for k in range(5)]
+    srv = O.Server(ctx, max_inflight_scans=2 * len(s), slice_photons=4096)
+    ids = [srv.submit(mid, p, s, seed + 20 + k) for k in range(5)]
+    outs = [srv._keep[rid] for rid in ids]  # the buffers the server writes (binding-owned)
+    srv.close()
+    for k, (out, n) in enumerate(outs):
+        assert np.array_equal(out.reshape(n, -1), refs[k]), k
+
+
 def test_translation_invariance_in_layers(ctx):
     # In a laterally uniform layered medium, A-scans away from the sides are statistically
     # the same (independent streams per A-scan).
