@@ -340,6 +340,51 @@ def test_close_finishes_queued_running_and_copying_requests(ctx, small_meshes):
         assert np.array_equal(out.reshape(n, -1), refs[k]), k
 
 
+def test_random_submit_cancel_release_interleaving(ctx, small_meshes):
+    # Requests of random meshes, seeds and A-scan subsets arrive while others finish, are
+    # copied back and are released; some are cancelled on the way. Every request ends: a
+    # cancelled one as cancelled (or done, if it finished first, then with the batched bits),
+    # every other with the batched bits; released ids are unknown afterwards.
+    rng = np.random.default_rng(5)
+    names = sorted(small_meshes)
+    srv = O.Server(ctx, slice_photons=2048, max_inflight_scans=24)
+    plan, results = [], {}
+
+    def take(rid):
+        info, out = srv.wait(rid, 120)
+        assert info.done
+        results[rid] = (info.status, out.copy())
+        srv.release(rid)
+        with pytest.raises(O.OctmpsError):
+            srv.poll(rid)
+
+    try:
+        live = []
+        for i in range(24):
+            name = names[int(rng.integers(len(names)))]
+            _, mid, p, s, seed = small_meshes[name]
+            pick = sorted(rng.choice(len(s), size=int(rng.integers(1, len(s) + 1)), replace=False))
+            scans = [s[j] for j in pick]
+            rid = srv.submit(mid, p, scans, seed + 100 + i)
+            cancel = bool(rng.random() < 0.25)
+            if cancel:
+                srv.cancel(rid)
+            plan.append((rid, mid, p, scans, seed + 100 + i, cancel))
+            live.append(rid)
+            if rng.random() < 0.3 and len(live) > 3:  # an earlier one ends and is released
+                take(live.pop(0))
+        for rid in live:
+            take(rid)
+    finally:
+        srv.close()
+    for rid, mid, p, scans, seed, cancel in plan:
+        status, out = results[rid]
+        if cancel and status == 10:
+            continue
+        assert status == 0, (rid, status)
+        assert np.array_equal(out, ctx.run_bscan(mid, p, scans, seed)[0]), rid
+
+
 def test_translation_invariance_in_layers(ctx):
     # In a laterally uniform layered medium, A-scans away from the sides are statistically
     # the same (independent streams per A-scan).
