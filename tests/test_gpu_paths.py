@@ -198,6 +198,21 @@ def test_served_slices_switch_kernel_variants_and_equal_batched(ctx, small_meshe
         ctx.release(mid10)
 
 
+@pytest.mark.parametrize("block", [64, 128, 192])
+def test_served_block_sizes_equal_batched(ctx, small_meshes, block):
+    # Blocks of fewer than 256 lanes use fewer of the cooperative fetch's per-wave stages (and
+    # a smaller grid per CU): the batched bits at every block size the server accepts.
+    refs = {k: ctx.run_bscan(v[1], v[2], v[3], v[4])[0] for k, v in small_meshes.items()}
+    srv = O.Server(ctx, slice_photons=4096, block_size=block)
+    try:
+        ids = {k: srv.submit(v[1], v[2], v[3], v[4]) for k, v in small_meshes.items()}
+        for k, rid in ids.items():
+            info, out = srv.wait(rid, 120)
+            assert info.status == 0 and np.array_equal(out, refs[k]), k
+    finally:
+        srv.close()
+
+
 def test_served_per_lane_record_path_equals_batched(ctx, small_meshes):
     # record_fetch 1 keeps every slice on the per-lane gathers, even where the cooperative
     # fetch fits; small slices make photons cross slice boundaries (save and restore).
