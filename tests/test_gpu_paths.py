@@ -72,7 +72,8 @@ def test_record_fetch_paths_give_the_same_bits(ctx, small_meshes, name):
     # launches and at every block size the launch accepts.
     m, mid, p, scans, seed = small_meshes[name]
     runs = []
-    for fetch, skip, block in ((1, 2, 256), (1, 1, 256), (2, 0, 256), (2, 0, 128), (2, 0, 192), (0, 0, 0)):
+    for fetch, skip, block in ((1, 2, 256), (1, 1, 256), (2, 0, 256), (2, 0, 64), (2, 0, 128), (2, 0, 192),
+                               (0, 0, 0)):
         o = O.BatchOpts()
         o.record_fetch, o.entry_skip, o.block_size = fetch, skip, block
         t, st = ctx.run_bscan(mid, p, scans, seed, opts=o)
