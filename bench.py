@@ -95,7 +95,9 @@ def parse():
     ap.add_argument("--serve-policy", type=int, default=0, choices=(0, 1),
                     help="request server hand-out order: 0 least remaining photons first, 1 equal shares")
     ap.add_argument("--serve-same", action="store_true",
-                    help="also submit the B-scan as one request to the server and compare its tallies")
+                    help="submit the B-scan as one request to the server and compare its tallies with the "
+                         "batched launch's (on by default, off with --no-serve unless this flag is given)")
+    ap.add_argument("--no-serve-same", action="store_true")
     ap.add_argument("--slab128-cells", type=int, default=128)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a plain copy kernel of this size first (byte-counter calibration)")
@@ -362,7 +364,7 @@ def main():
         serving = run_serving(O, configs, ctx, a)
 
     served_same = None
-    if a.serve_same:
+    if a.serve_same or not (a.no_serve or a.no_serve_same):
         # The same B-scan as one request to the request server (the multi-set kernel in
         # wall-clock slices): its rate beside the batched launch and its tallies against the
         # batched launch's, bit for bit (the batched tallies come from the timed steps below).
